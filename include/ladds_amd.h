@@ -1,0 +1,223 @@
+/*
+ * ladds_amd.h -- C ABI of the MI355X (gfx950) propagate + conjunction engine.
+ *
+ * This is the drop-in boundary for LADDS's per-timestep hot path.  Every entry
+ * point lists the reference interface it replaces (paths relative to the LADDS
+ * tree; OrbitPropagator and AutoPas are the pinned upstream dependencies that
+ * LADDS fetches at build time):
+ *
+ *   YoshidaIntegrator<AutoPas_t>(autopas, accumulator, deltaT)    src/ladds/Simulation.cpp:145
+ *   YoshidaIntegrator::integrate(false)                             src/ladds/Simulation.cpp:297
+ *   AccelerationAccumulator<AutoPas_t>(flags[8], autopas, 0.0, w)   src/ladds/Simulation.cpp:127-143
+ *   AutoPas<Particle>::iteratePairwise(&CollisionFunctor)            src/ladds/Simulation.cpp:220
+ *   CollisionFunctor(cutoff, dt, cdf, minDetRadius, evasionCutoff)  src/ladds/CollisionFunctor.cpp:18-30
+ *   CollisionFunctor::getCollisions / getEvadedCollisions           src/ladds/CollisionFunctor.cpp:32-38
+ *   Simulation::simulationLoop (integrate + collision detection)    src/ladds/Simulation.cpp:287-415
+ *   Simulation::deleteBurnUps                                       src/ladds/Simulation.cpp:561-596
+ *
+ * Conventions
+ *   - Units follow LADDS: km, km/s, s; radius in m; aom in km^2/kg; bc_inv in m^2/kg.
+ *   - Time t is seconds since J2000.0; the accumulator start time of the reference
+ *     (Simulation.cpp:143, 0.0) is ladds_config.t0.
+ *   - All entry points return LADDS_OK (0) or a negative status.  No C++ exception
+ *     crosses this boundary; ladds_last_error() holds the message.
+ *   - A context owns its device memory and one HIP stream, and is bound to one host
+ *     thread at a time (not re-entrant), as the reference's single-threaded driver.
+ *   - Pairs carry ids and object indices (positions in the array last passed to
+ *     ladds_set_objects, after any burn-up compaction) instead of Particle*; a host
+ *     adapter maps indices back to its own objects.
+ */
+#ifndef LADDS_AMD_H
+#define LADDS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LADDS_ABI_VERSION 1
+
+#if defined(__GNUC__)
+#define LADDS_API __attribute__((visibility("default")))
+#else
+#define LADDS_API
+#endif
+
+/* ---- status codes ---- */
+#define LADDS_OK 0
+#define LADDS_E_INVALID (-1)   /* bad argument                                       */
+#define LADDS_E_HIP (-2)       /* HIP runtime error                                  */
+#define LADDS_E_OVERFLOW (-3)  /* output capacity too small; *n holds required count */
+#define LADDS_E_NOMEM (-4)     /* device allocation failed                           */
+#define LADDS_E_STATE (-5)     /* call out of order (e.g. no objects set)            */
+#define LADDS_E_NODEVICE (-6)  /* no usable gfx950 device                            */
+
+/* ---- force components, bit order of the flag array at src/ladds/Simulation.cpp:127-135 ---- */
+#define LADDS_KEP (1u << 0)
+#define LADDS_J2 (1u << 1)
+#define LADDS_C22 (1u << 2)
+#define LADDS_S22 (1u << 3)
+#define LADDS_SOL (1u << 4)
+#define LADDS_LUN (1u << 5)
+#define LADDS_SRP (1u << 6)
+#define LADDS_DRAG (1u << 7)
+#define LADDS_ALL_COMPONENTS 0xFFu
+
+/* ---- precision forms ---- */
+#define LADDS_FP64 0       /* every stage in FP64                                                   */
+#define LADDS_SPLIT 1      /* FP64: state, Kepler, J2, closest approach; FP32: C22/S22, Sun, Moon,  */
+                           /* SRP, drag and the conservative conjunction prefilter                  */
+#define LADDS_SPLIT_ALL32 2 /* as LADDS_SPLIT with J2 also in FP32 (larger position error)          */
+
+/* ---- conjunction threshold modes ---- */
+#define LADDS_THRESHOLD_RADIUS_SCALED 0 /* reference: d^2 <= ((R_i+R_j) cdf / 1000)^2, CollisionFunctor.cpp:85,112 */
+#define LADDS_THRESHOLD_ABSOLUTE 1      /* d^2 <= threshold_km^2 (no reference counterpart) */
+
+/* ---- ActivityState, src/ladds/particle/Particle.h:37-53 ---- */
+#define LADDS_PASSIVE 0
+#define LADDS_EVASIVE 1
+#define LADDS_EVASIVE_PRESERVING 2
+
+/* ---- pair-scan output modes ---- */
+#define LADDS_PAIRS_LIST 0   /* write pair records                                  */
+#define LADDS_PAIRS_DIGEST 1 /* count + order-independent 128-bit digest only       */
+
+typedef struct ladds_ctx ladds_ctx;
+
+typedef struct ladds_config {
+  double dt;                         /* sim/deltaT [s]                                        */
+  double t0;                         /* accumulator start time [s since J2000]                */
+  uint32_t components;               /* sim/prop/use{KEP..DRAG}Component bits                  */
+  int32_t precision;                 /* LADDS_FP64 | LADDS_SPLIT | LADDS_SPLIT_ALL32          */
+  double cutoff_km;                  /* autopas/cutoff                                        */
+  double collision_distance_factor;  /* sim/collisionDistanceFactor                           */
+  double min_detection_radius_m;     /* sim/minDetectionRadius                                */
+  double evasion_cutoff_km;          /* sim/evasionTrackingCutoffInKM                         */
+  int32_t threshold_mode;            /* LADDS_THRESHOLD_*                                     */
+  double threshold_km;               /* absolute mode threshold                               */
+  double cell_size_km;               /* 0: 2 * cutoff (minimum allowed)                        */
+  int32_t scan_variant;              /* 0: auto/autotuned, 1: global-memory scan, 2: LDS tiles */
+  int32_t block_size;                /* pair-scan workgroup size, 0 = auto                      */
+  int32_t device;                    /* HIP device ordinal                                    */
+  int32_t pair_output;               /* ladds_simulate: LADDS_PAIRS_LIST | LADDS_PAIRS_DIGEST  */
+  uint64_t log_capacity;             /* pair log entries kept on device by ladds_simulate     */
+} ladds_config;
+
+/* Object arrays, structure-of-arrays, host memory; copied in (caller keeps ownership). */
+typedef struct ladds_objects {
+  size_t n;
+  const double *rx, *ry, *rz;   /* position [km]                                  */
+  const double *vx, *vy, *vz;   /* velocity [km/s]                                */
+  const double *radius_m;       /* Particle::_radius [m]                          */
+  const double *aom;            /* Particle::_aom [km^2/kg]   (NULL: 0)           */
+  const double *bc_inv;         /* Particle::_bc_inv [m^2/kg] (NULL: 0)           */
+  const double *cr;             /* radiation pressure coefficient (NULL: 1.0)     */
+  const int32_t *activity;      /* ActivityState (NULL: passive)                  */
+  const uint64_t *id;           /* Particle id (NULL: index)                      */
+  const uint64_t *parent;       /* parentIdentifier (NULL: UINT64_MAX)            */
+} ladds_objects;
+
+/* One conjunction: ids ordered as CollisionFunctor.cpp:94-128 (smaller id first). */
+typedef struct ladds_pair {
+  uint64_t id_a, id_b;
+  uint32_t idx_a, idx_b;
+  double d2;      /* squared closest-approach distance [km^2] */
+  double mid[3];  /* collision point p2 + dr/2 [km]          */
+} ladds_pair;     /* 56 bytes */
+
+/* A pair recorded by ladds_simulate, tagged with its iteration. */
+typedef struct ladds_logged_pair {
+  ladds_pair pair;
+  uint64_t iteration;
+  uint32_t evaded; /* 0 collision, 1 evaded conjunction */
+  uint32_t reserved;
+} ladds_logged_pair; /* 72 bytes */
+
+typedef struct ladds_run_stats {
+  uint64_t collisions;      /* total collisions found (also beyond log capacity) */
+  uint64_t evasions;        /* total evaded conjunctions                          */
+  uint64_t logged;          /* entries stored in the device log                  */
+  uint64_t candidates;      /* pairs that passed the cutoff prefilter             */
+  uint64_t digest[2];       /* order-independent digest of collision id pairs     */
+  uint64_t edigest[2];      /* same for evaded conjunctions                        */
+  uint64_t remaining;       /* objects left after burn-up compaction               */
+} ladds_run_stats;
+
+/* ---- library ---- */
+LADDS_API const char *ladds_version(void);
+LADDS_API int ladds_abi_version(void);
+LADDS_API int ladds_device_count(int *count);
+LADDS_API void ladds_config_default(ladds_config *cfg);
+
+/* ---- context: replaces the (csvWriter, accumulator, integrator) tuple built by
+ *      Simulation::initIntegrator (Simulation.cpp:122-148) plus the AutoPas container
+ *      built by Simulation::initAutoPas (Simulation.cpp:59-120) ---- */
+LADDS_API int ladds_create(const ladds_config *cfg, ladds_ctx **out);
+LADDS_API int ladds_destroy(ladds_ctx *ctx);
+LADDS_API const char *ladds_last_error(const ladds_ctx *ctx);
+
+/* AutoPas::addParticle for a whole population (SatelliteLoader.cpp:17-84). Replaces all objects. */
+LADDS_API int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *objs);
+LADDS_API size_t ladds_num_objects(const ladds_ctx *ctx);
+/* Read back state (caller-owned arrays of ladds_num_objects() entries; NULL skips). */
+LADDS_API int ladds_get_state(ladds_ctx *ctx, double *rx, double *ry, double *rz, double *vx, double *vy, double *vz);
+/* Ids of the current objects, in index order (after compaction). */
+LADDS_API int ladds_get_ids(ladds_ctx *ctx, uint64_t *id);
+LADDS_API double ladds_get_time(const ladds_ctx *ctx);
+LADDS_API int ladds_set_time(ladds_ctx *ctx, double t);
+
+/* YoshidaIntegrator::integrate(false), nsteps times (Simulation.cpp:297). */
+LADDS_API int ladds_propagate(ladds_ctx *ctx, int nsteps);
+
+/* Simulation::collisionDetection (Simulation.cpp:211-222): AutoPas::iteratePairwise with a
+ * CollisionFunctor(cutoff, dt_interp, cdf, minDetectionRadius, evasionCutoff) on the current
+ * state.  Unordered.  On overflow returns LADDS_E_OVERFLOW with *n / *ne = required counts. */
+LADDS_API int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_t cap, size_t *n,
+                       ladds_pair *evaded, size_t ecap, size_t *ne);
+
+/* Count + order-independent 128-bit digest of the collision and evasion sets on the current
+ * state, without materialising pairs (for populations whose pair sets do not fit memory). */
+LADDS_API int ladds_conjunction_digest(ladds_ctx *ctx, double dt_interp, uint64_t *n, uint64_t digest[2], uint64_t *ne,
+                             uint64_t edigest[2]);
+
+/* Simulation::simulationLoop hot path (Simulation.cpp:287-415) for iterations
+ * [first_iteration, first_iteration + nsteps): integrate; deleteBurnUps if
+ * min_altitude_km > 0 (Simulation.cpp:301-303); collision detection when
+ * iteration % tpcd == 0 with window dt*tpcd (Simulation.cpp:389-397).  Pairs are
+ * appended to the device log (ladds_fetch_log); totals in *stats (may be NULL). */
+LADDS_API int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpcd, double min_altitude_km,
+                   ladds_run_stats *stats);
+LADDS_API int ladds_fetch_log(ladds_ctx *ctx, ladds_logged_pair *out, size_t cap, size_t *n);
+LADDS_API int ladds_clear_log(ladds_ctx *ctx);
+
+/* Simulation::deleteBurnUps (Simulation.cpp:561-596): remove objects with |r| < R_E + h_min,
+ * compacting the arrays on the device (order preserved). */
+LADDS_API int ladds_delete_burnups(ladds_ctx *ctx, double min_altitude_km, size_t *removed);
+
+/* ---- measurement ---- */
+/* Record HIP events around every kernel launch; ladds_kernel_stats reports per kernel kind. */
+LADDS_API int ladds_set_profiling(ladds_ctx *ctx, int enable);
+/* name: "propagate", "bin_reduce", "bin_scan", "bin_scatter", "pair_scan"; returns launches and
+ * summed device milliseconds since profiling was enabled (synchronises). */
+LADDS_API int ladds_kernel_stats(ladds_ctx *ctx, const char *name, uint64_t *launches, double *total_ms);
+LADDS_API int ladds_synchronize(ladds_ctx *ctx);
+/* HIP stream the context launches on (hipStream_t as void*). */
+LADDS_API void *ladds_stream(ladds_ctx *ctx);
+
+/* ---- seeded synthetic populations (host only, no device needed) ----
+ * workload 0: Keplerian elements as the 16,384-object case (perigee altitude U[200,2000] km,
+ *             e U[0,0.02], inclination mixture, radius log-U[0.05,5] m, bc_inv from c_D = 2.2);
+ * workload 1: as 0 plus a 1-cm debris tail (90% radius log-U[0.005,0.1] m), bc_inv U[0.005,0.05],
+ *             Cr U[1.0,1.5];
+ * workload 2: 60% constellation shells replicated with +-50 m jitter, 40% as workload 0.
+ * planted_pairs > 0 appends co-orbital / crossing pairs 0.3-0.9 km apart (for tests). */
+LADDS_API int ladds_generate_population(int workload, uint64_t seed, size_t n, size_t planted_pairs, double *rx, double *ry,
+                              double *rz, double *vx, double *vy, double *vz, double *radius_m, double *mass_kg,
+                              double *aom, double *bc_inv, double *cr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,294 @@
+"""ladds_amd -- MI355X (gfx950) propagate + conjunction engine for LADDS.
+
+Python mirror of the reference's hot-path call surface, over the C ABI of
+``include/ladds_amd.h`` (``libladds_amd.so``):
+
+=========================================  ================================================
+reference (LADDS / OrbitPropagator)         here
+=========================================  ================================================
+``Simulation::initAutoPas`` + container     :class:`Engine` (owns device state)
+``AutoPas::addParticle`` (bulk)             :meth:`Engine.set_objects`
+``YoshidaIntegrator::integrate(false)``     :meth:`Engine.propagate`
+``Simulation::collisionDetection``          :meth:`Engine.conjunctions`
+``Simulation::simulationLoop`` hot path     :meth:`Engine.simulate`
+``Simulation::deleteBurnUps``               :meth:`Engine.delete_burnups`
+=========================================  ================================================
+
+Errors are raised as :class:`LaddsError` (the reference throws ``std::runtime_error``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from ._abi import (ALL_COMPONENTS, C22, DRAG, EVASIVE, EVASIVE_PRESERVING, FP64, J2, KEP, LUN, PAIRS_DIGEST,
+                   PAIRS_LIST, PASSIVE, S22, SOL, SPLIT, SPLIT_ALL32, SRP, THRESHOLD_ABSOLUTE,
+                   THRESHOLD_RADIUS_SCALED, LOGGED_DTYPE, PAIR_DTYPE)
+
+__all__ = [
+    "Engine", "EngineConfig", "LaddsError", "generate_population", "device_count", "components_mask",
+    "KEP", "J2", "C22", "S22", "SOL", "LUN", "SRP", "DRAG", "ALL_COMPONENTS", "FP64", "SPLIT", "SPLIT_ALL32",
+    "THRESHOLD_RADIUS_SCALED", "THRESHOLD_ABSOLUTE", "PASSIVE", "EVASIVE", "EVASIVE_PRESERVING",
+    "PAIRS_LIST", "PAIRS_DIGEST", "PAIR_DTYPE", "LOGGED_DTYPE",
+]
+
+lib = _abi.lib
+
+
+class LaddsError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"ladds error {code}: {message}")
+        self.code = code
+
+
+def components_mask(kep=False, j2=False, c22=False, s22=False, sol=False, lun=False, srp=False, drag=False) -> int:
+    """Bit mask from the ``sim/prop/use*Component`` flags (Simulation.cpp:127-135)."""
+    flags = (kep, j2, c22, s22, sol, lun, srp, drag)
+    return sum(1 << k for k, f in enumerate(flags) if f)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    lib.ladds_device_count(C.byref(n))
+    return n.value
+
+
+@dataclass
+class EngineConfig:
+    """Mirror of ``ladds_config``; field names follow the reference's YAML keys."""
+    dt: float = 10.0                      # sim/deltaT
+    t0: float = 0.0                       # accumulator start time (Simulation.cpp:143)
+    components: int = KEP                 # sim/prop/use*Component
+    precision: int = FP64
+    cutoff_km: float = 80.0               # autopas/cutoff
+    collision_distance_factor: float = 1.0  # sim/collisionDistanceFactor
+    min_detection_radius_m: float = 0.05  # sim/minDetectionRadius
+    evasion_cutoff_km: float = 0.1        # sim/evasionTrackingCutoffInKM
+    threshold_mode: int = THRESHOLD_RADIUS_SCALED
+    threshold_km: float = 1.0
+    cell_size_km: float = 0.0
+    scan_variant: int = 0
+    block_size: int = 0
+    device: int = 0
+    pair_output: int = PAIRS_LIST
+    log_capacity: int = 1 << 20
+
+    def to_c(self) -> _abi.Config:
+        c = _abi.Config()
+        lib.ladds_config_default(C.byref(c))
+        for name, _ in _abi.Config._fields_:
+            setattr(c, name, getattr(self, name))
+        return c
+
+
+def _ptr(a, ctype=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _f64(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape != (n,):
+        raise ValueError(f"expected shape ({n},), got {a.shape}")
+    return a
+
+
+@dataclass
+class Population:
+    rx: np.ndarray
+    ry: np.ndarray
+    rz: np.ndarray
+    vx: np.ndarray
+    vy: np.ndarray
+    vz: np.ndarray
+    radius_m: np.ndarray
+    mass_kg: np.ndarray
+    aom: np.ndarray
+    bc_inv: np.ndarray
+    cr: np.ndarray
+    activity: np.ndarray = field(default=None)
+    id: np.ndarray = field(default=None)
+    parent: np.ndarray = field(default=None)
+
+    @property
+    def n(self) -> int:
+        return self.rx.shape[0]
+
+    def r(self) -> np.ndarray:
+        return np.stack([self.rx, self.ry, self.rz], 1)
+
+    def v(self) -> np.ndarray:
+        return np.stack([self.vx, self.vy, self.vz], 1)
+
+    def copy(self) -> "Population":
+        return Population(**{k: (None if v is None else v.copy()) for k, v in self.__dict__.items()})
+
+
+def generate_population(workload: int, seed: int, n: int, planted_pairs: int = 0) -> Population:
+    """Seeded synthetic population (``ladds_generate_population``); host only."""
+    arrs = [np.empty(n, np.float64) for _ in range(11)]
+    rc = lib.ladds_generate_population(workload, seed, n, planted_pairs, *[_ptr(a) for a in arrs])
+    if rc != 0:
+        raise LaddsError(rc, "ladds_generate_population")
+    pop = Population(*arrs)
+    pop.activity = np.zeros(n, np.int32)
+    pop.id = np.arange(n, dtype=np.uint64)
+    pop.parent = np.full(n, np.iinfo(np.uint64).max, np.uint64)
+    return pop
+
+
+class Engine:
+    """One device context: the AutoPas container + Yoshida integrator + CollisionFunctor."""
+
+    def __init__(self, config: EngineConfig | None = None, **kw):
+        self.config = config or EngineConfig(**kw)
+        self._c = self.config.to_c()
+        h = C.c_void_p()
+        rc = lib.ladds_create(C.byref(self._c), C.byref(h))
+        if rc != 0:
+            raise LaddsError(rc, "ladds_create failed (no usable gfx950 device?)")
+        self._h = h
+
+    # -- lifetime -------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ladds_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = lib.ladds_last_error(self._h)
+            raise LaddsError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    # -- state ----------------------------------------------------------
+    def set_objects(self, pop: Population | None = None, **arrays):
+        """``AutoPas::addParticle`` for a whole population (replaces all objects)."""
+        src = dict(pop.__dict__) if pop is not None else {}
+        src.update(arrays)
+        n = len(src["rx"])
+        keep = {}
+        o = _abi.Objects()
+        o.n = n
+        for name in ("rx", "ry", "rz", "vx", "vy", "vz", "radius_m"):
+            keep[name] = _f64(src[name], n)
+            setattr(o, name, _ptr(keep[name]))
+        for name in ("aom", "bc_inv", "cr"):
+            if src.get(name) is not None:
+                keep[name] = _f64(src[name], n)
+                setattr(o, name, _ptr(keep[name]))
+        if src.get("activity") is not None:
+            keep["activity"] = np.ascontiguousarray(src["activity"], dtype=np.int32)
+            o.activity = _ptr(keep["activity"], C.c_int32)
+        for name in ("id", "parent"):
+            if src.get(name) is not None:
+                keep[name] = np.ascontiguousarray(src[name], dtype=np.uint64)
+                setattr(o, name, _ptr(keep[name], C.c_uint64))
+        self._check(lib.ladds_set_objects(self._h, C.byref(o)), "set_objects")
+
+    @property
+    def n(self) -> int:
+        return int(lib.ladds_num_objects(self._h))
+
+    @property
+    def time(self) -> float:
+        return float(lib.ladds_get_time(self._h))
+
+    def set_time(self, t: float):
+        self._check(lib.ladds_set_time(self._h, t), "set_time")
+
+    def state(self):
+        """(r, v) as (n, 3) arrays."""
+        n = self.n
+        a = [np.empty(n, np.float64) for _ in range(6)]
+        self._check(lib.ladds_get_state(self._h, *[_ptr(x) for x in a]), "get_state")
+        return np.stack(a[:3], 1), np.stack(a[3:], 1)
+
+    def ids(self) -> np.ndarray:
+        out = np.empty(self.n, np.uint64)
+        self._check(lib.ladds_get_ids(self._h, _ptr(out, C.c_uint64)), "get_ids")
+        return out
+
+    # -- hot path -------------------------------------------------------
+    def propagate(self, nsteps: int = 1):
+        """``YoshidaIntegrator::integrate(false)`` nsteps times."""
+        self._check(lib.ladds_propagate(self._h, int(nsteps)), "propagate")
+
+    def conjunctions(self, dt_interp: float | None = None):
+        """``Simulation::collisionDetection``: (collisions, evasions) as PAIR_DTYPE arrays."""
+        dt = self.config.dt if dt_interp is None else dt_interp
+        cap = 1024
+        while True:
+            hits = np.zeros(cap, PAIR_DTYPE)
+            ev = np.zeros(cap, PAIR_DTYPE)
+            n, ne = C.c_size_t(0), C.c_size_t(0)
+            rc = lib.ladds_conjunctions(self._h, dt, hits.ctypes.data, cap, C.byref(n), ev.ctypes.data, cap,
+                                        C.byref(ne))
+            if rc == _abi.E_OVERFLOW:
+                cap = max(n.value, ne.value) + 1
+                continue
+            self._check(rc, "conjunctions")
+            return hits[: n.value], ev[: ne.value]
+
+    def conjunction_digest(self, dt_interp: float | None = None):
+        dt = self.config.dt if dt_interp is None else dt_interp
+        n, ne = C.c_uint64(0), C.c_uint64(0)
+        d = (C.c_uint64 * 2)()
+        e = (C.c_uint64 * 2)()
+        self._check(lib.ladds_conjunction_digest(self._h, dt, C.byref(n), d, C.byref(ne), e), "conjunction_digest")
+        return n.value, (d[0], d[1]), ne.value, (e[0], e[1])
+
+    def simulate(self, nsteps: int, first_iteration: int = 0, tpcd: int = 1, min_altitude_km: float = 0.0):
+        """``Simulation::simulationLoop`` hot path; returns the run statistics as a dict."""
+        st = _abi.RunStats()
+        self._check(lib.ladds_simulate(self._h, int(first_iteration), int(nsteps), int(tpcd),
+                                       float(min_altitude_km), C.byref(st)), "simulate")
+        return {"collisions": st.collisions, "evasions": st.evasions, "logged": st.logged,
+                "candidates": st.candidates, "digest": (st.digest[0], st.digest[1]),
+                "edigest": (st.edigest[0], st.edigest[1]), "remaining": st.remaining}
+
+    def fetch_log(self) -> np.ndarray:
+        n = C.c_size_t(0)
+        rc = lib.ladds_fetch_log(self._h, None, 0, C.byref(n))
+        if rc not in (0, _abi.E_OVERFLOW):
+            self._check(rc, "fetch_log")
+        out = np.zeros(max(n.value, 1), LOGGED_DTYPE)
+        self._check(lib.ladds_fetch_log(self._h, out.ctypes.data, out.shape[0], C.byref(n)), "fetch_log")
+        return out[: n.value]
+
+    def clear_log(self):
+        self._check(lib.ladds_clear_log(self._h), "clear_log")
+
+    def delete_burnups(self, min_altitude_km: float) -> int:
+        removed = C.c_size_t(0)
+        self._check(lib.ladds_delete_burnups(self._h, float(min_altitude_km), C.byref(removed)), "delete_burnups")
+        return removed.value
+
+    # -- measurement ----------------------------------------------------
+    def set_profiling(self, enable: bool):
+        self._check(lib.ladds_set_profiling(self._h, int(bool(enable))), "set_profiling")
+
+    def kernel_stats(self, name: str):
+        launches, ms = C.c_uint64(0), C.c_double(0.0)
+        self._check(lib.ladds_kernel_stats(self._h, name.encode(), C.byref(launches), C.byref(ms)), "kernel_stats")
+        return launches.value, ms.value
+
+    def synchronize(self):
+        self._check(lib.ladds_synchronize(self._h), "synchronize")
+
+    @property
+    def stream(self) -> int:
+        return int(lib.ladds_stream(self._h) or 0)

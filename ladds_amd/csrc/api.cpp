@@ -1,0 +1,782 @@
+// api.cpp -- the C ABI (include/ladds_amd.h): context, memory, and the per-step schedule.
+//
+// A context replaces the objects Simulation::run builds (src/ladds/Simulation.cpp:522-549):
+// the AutoPas container (initAutoPas :59-120), the accumulator + Yoshida integrator
+// (initIntegrator :122-148) and the CollisionFunctor built per detection
+// (collisionDetection :211-222).  All state lives in HBM; host calls only enqueue
+// kernels on the context's stream, except where results are returned.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ladds_amd.h"
+#include "constants.h"
+#include "engine.h"
+#include "ephemeris.h"
+#include "launch.h"
+
+using namespace ladds;
+
+namespace {
+
+struct EventPair {
+  hipEvent_t a, b;
+};
+
+struct KernelProfile {
+  std::vector<EventPair> used;
+  uint64_t launches = 0;
+  double ms = 0.0;  // resolved time
+};
+
+constexpr const char *kKernelNames[] = {"propagate", "bin_count", "bin_reduce",  "bin_scan",
+                                        "bin_scatter", "pair_scan", "compact"};
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * count);
+}
+
+uint32_t ceil_log2(uint64_t v) {
+  uint32_t b = 0;
+  while ((1ull << b) < v) ++b;
+  return b;
+}
+
+}  // namespace
+
+struct ladds_ctx {
+  ladds_config cfg{};
+  int form = kFormFp64;
+  std::string err;
+  hipStream_t stream = nullptr;
+  int device = 0;
+
+  size_t n = 0, cap = 0;
+  DeviceObjects d{};
+  DeviceObjects d_alt{};  // compaction target
+
+  GridParams grid{};
+  uint32_t *count = nullptr, *slot = nullptr, *start = nullptr, *tile_sum = nullptr, *sidx = nullptr;
+  void *rec = nullptr;
+  uint32_t *n_alive = nullptr;
+  uint32_t grid_buckets_alloc = 0;
+
+  SubstepEnv<double> *env64 = nullptr;
+  SubstepEnv<float> *env32 = nullptr;
+  size_t env_cap = 0;  // sub-steps
+  std::vector<SubstepEnv<double>> h_env64;
+  std::vector<SubstepEnv<float>> h_env32;
+
+  LoggedPair *log = nullptr;
+  uint64_t log_cap = 0;
+  unsigned long long *log_count = nullptr;
+  ScanCounters *counters = nullptr;
+
+  uint64_t step_count = 0;  // steps since t0
+  double c[4], dcoef[3];
+
+  bool profiling = false;
+  std::map<std::string, KernelProfile> prof;
+  std::vector<EventPair> event_pool;
+
+  int fail(int code, const std::string &msg) {
+    err = msg;
+    return code;
+  }
+  int hipfail(hipError_t e, const char *what) {
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return (e == hipErrorOutOfMemory) ? LADDS_E_NOMEM : LADDS_E_HIP;
+  }
+  double time() const { return cfg.t0 + (double)step_count * cfg.dt; }
+
+  EventPair get_events() {
+    if (!event_pool.empty()) {
+      EventPair p = event_pool.back();
+      event_pool.pop_back();
+      return p;
+    }
+    EventPair p{};
+    hipEventCreate(&p.a);
+    hipEventCreate(&p.b);
+    return p;
+  }
+};
+
+#define HIPCHK(ctx, call)                                 \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
+    if (e_ != hipSuccess) return (ctx)->hipfail(e_, #call); \
+  } while (0)
+
+namespace {
+
+// Launch wrapper: records events around the launch when profiling is on.
+template <typename F>
+hipError_t timed(ladds_ctx *ctx, const char *name, F &&launch) {
+  if (!ctx->profiling) return launch();
+  EventPair ev = ctx->get_events();
+  (void)hipEventRecord(ev.a, ctx->stream);
+  const hipError_t e = launch();
+  (void)hipEventRecord(ev.b, ctx->stream);
+  KernelProfile &kp = ctx->prof[name];
+  kp.used.push_back(ev);
+  kp.launches += 1;
+  return e;
+}
+
+void resolve_profiles(ladds_ctx *ctx) {
+  hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->prof) {
+    for (EventPair &ev : kv.second.used) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess) kv.second.ms += ms;
+      ctx->event_pool.push_back(ev);
+    }
+    kv.second.used.clear();
+  }
+}
+
+void free_objects(DeviceObjects &o) {
+  hipFree(o.rx);
+  hipFree(o.ry);
+  hipFree(o.rz);
+  hipFree(o.vx);
+  hipFree(o.vy);
+  hipFree(o.vz);
+  hipFree(o.radius_m);
+  hipFree(o.aom);
+  hipFree(o.bc_inv);
+  hipFree(o.cr);
+  hipFree(o.srp64);
+  hipFree(o.drag64);
+  hipFree(o.srp32);
+  hipFree(o.drag32);
+  hipFree(o.activity);
+  hipFree(o.id);
+  hipFree(o.parent);
+  o = DeviceObjects{};
+}
+
+hipError_t alloc_objects(DeviceObjects &o, size_t cap) {
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) {
+    if (e == hipSuccess) e = x;
+  };
+  chk(dalloc(&o.rx, cap));
+  chk(dalloc(&o.ry, cap));
+  chk(dalloc(&o.rz, cap));
+  chk(dalloc(&o.vx, cap));
+  chk(dalloc(&o.vy, cap));
+  chk(dalloc(&o.vz, cap));
+  chk(dalloc(&o.radius_m, cap));
+  chk(dalloc(&o.aom, cap));
+  chk(dalloc(&o.bc_inv, cap));
+  chk(dalloc(&o.cr, cap));
+  chk(dalloc(&o.srp64, cap));
+  chk(dalloc(&o.drag64, cap));
+  chk(dalloc(&o.srp32, cap));
+  chk(dalloc(&o.drag32, cap));
+  chk(dalloc(&o.activity, cap));
+  chk(dalloc(&o.id, cap));
+  chk(dalloc(&o.parent, cap));
+  return e;
+}
+
+int setup_grid(ladds_ctx *ctx, size_t n) {
+  const double min_cell = 2.0 * ctx->cfg.cutoff_km * (1.0 + 1e-9);
+  double cell = ctx->cfg.cell_size_km > 0.0 ? ctx->cfg.cell_size_km : min_cell;
+  if (!(cell >= min_cell)) return ctx->fail(LADDS_E_INVALID, "cell_size_km must be >= 2 * cutoff_km");
+  const uint32_t bits = std::min<uint32_t>(28u, std::max<uint32_t>(12u, ceil_log2(std::max<size_t>(n, 1))));
+  GridParams g{};
+  g.inv_cell = 1.0 / cell;
+  g.bx = (bits + 2) / 3;
+  g.by = (bits + 1) / 3;
+  g.bz = bits / 3;
+  g.nbuckets = 1u << bits;
+  ctx->grid = g;
+  if (g.nbuckets > ctx->grid_buckets_alloc) {
+    hipFree(ctx->count);
+    hipFree(ctx->start);
+    hipFree(ctx->tile_sum);
+    HIPCHK(ctx, dalloc(&ctx->count, g.nbuckets));
+    HIPCHK(ctx, dalloc(&ctx->start, (size_t)g.nbuckets + 1));
+    HIPCHK(ctx, dalloc(&ctx->tile_sum, g.nbuckets / 2048 + 1));
+    HIPCHK(ctx, hipMemset(ctx->count, 0, sizeof(uint32_t) * g.nbuckets));
+    ctx->grid_buckets_alloc = g.nbuckets;
+  }
+  return LADDS_OK;
+}
+
+FunctorParams functor_params(const ladds_ctx *ctx, double dt_interp) {
+  const ladds_config &c = ctx->cfg;
+  FunctorParams fp{};
+  fp.cutoff2 = c.cutoff_km * c.cutoff_km;
+  fp.cdf_km = c.collision_distance_factor / 1000.;
+  fp.evasion2 = c.evasion_cutoff_km * c.evasion_cutoff_km;
+  fp.min_det_radius = c.min_detection_radius_m;
+  fp.threshold2 = c.threshold_km * c.threshold_km;
+  fp.dt = dt_interp;
+  fp.threshold_mode = c.threshold_mode;
+  // FP32 prefilter radius: positions rounded to FP32 are off by <= 2^-24 |x| per axis;
+  // for |x| < 1e5 km the distance error is < 0.014 km, covered by +0.1 km.
+  const double inflated = c.cutoff_km * (1.0 + 1e-5) + 0.1;
+  fp.cutoff2_f32 = std::nextafter((float)(inflated * inflated), INFINITY);
+  return fp;
+}
+
+int ensure_env(ladds_ctx *ctx, size_t nsteps) {
+  const size_t need = 3 * nsteps;
+  if (need > ctx->env_cap) {
+    hipFree(ctx->env64);
+    hipFree(ctx->env32);
+    HIPCHK(ctx, dalloc(&ctx->env64, need));
+    HIPCHK(ctx, dalloc(&ctx->env32, need));
+    ctx->env_cap = need;
+  }
+  ctx->h_env64.resize(need);
+  ctx->h_env32.resize(need);
+  const double tau[3] = {ctx->c[0], ctx->c[0] + ctx->c[1], ctx->c[0] + ctx->c[1] + ctx->c[2]};
+  for (size_t s = 0; s < nsteps; ++s) {
+    const double t = ctx->cfg.t0 + (double)(ctx->step_count + s) * ctx->cfg.dt;
+    for (int k = 0; k < 3; ++k) substep_env(t + tau[k] * ctx->cfg.dt, ctx->h_env64[3 * s + k], ctx->h_env32[3 * s + k]);
+  }
+  HIPCHK(ctx, hipMemcpyAsync(ctx->env64, ctx->h_env64.data(), sizeof(SubstepEnv<double>) * need,
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->env32, ctx->h_env32.data(), sizeof(SubstepEnv<float>) * need,
+                             hipMemcpyHostToDevice, ctx->stream));
+  return LADDS_OK;
+}
+
+int ensure_log(ladds_ctx *ctx, uint64_t cap) {
+  if (cap <= ctx->log_cap) return LADDS_OK;
+  LoggedPair *nl = nullptr;
+  HIPCHK(ctx, dalloc(&nl, cap));
+  if (ctx->log) {
+    HIPCHK(ctx, hipMemcpyAsync(nl, ctx->log, sizeof(LoggedPair) * ctx->log_cap, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(ctx->log);
+  }
+  ctx->log = nl;
+  ctx->log_cap = cap;
+  return LADDS_OK;
+}
+
+int launch_propagate_steps(ladds_ctx *ctx, size_t env_offset, int nsteps, bool count, double burnup_r2) {
+  double cdt[4], ddt[3];
+  for (int k = 0; k < 4; ++k) cdt[k] = ctx->c[k] * ctx->cfg.dt;
+  for (int k = 0; k < 3; ++k) ddt[k] = ctx->dcoef[k] * ctx->cfg.dt;
+  const hipError_t e = timed(ctx, "propagate", [&] {
+    return launch_propagate(ctx->form, ctx->d, ctx->env64 + 3 * env_offset, ctx->env32 + 3 * env_offset, nsteps,
+                            ctx->cfg.components, cdt, ddt, &ctx->grid, count ? ctx->count : nullptr, ctx->slot,
+                            burnup_r2, ctx->stream);
+  });
+  if (e != hipSuccess) return ctx->hipfail(e, "propagate launch");
+  return LADDS_OK;
+}
+
+// Binning (after the count pass) + pair scan on the current state.
+int launch_detection(ladds_ctx *ctx, bool counted, double dt_interp, bool digest, uint64_t iteration) {
+  const FunctorParams fp = functor_params(ctx, dt_interp);
+  hipError_t e = hipSuccess;
+  if (!counted)
+    e = timed(ctx, "bin_count", [&] { return launch_bin_count(ctx->d, ctx->grid, ctx->count, ctx->slot, ctx->stream); });
+  if (e == hipSuccess)
+    e = timed(ctx, "bin_reduce", [&] { return launch_bin_reduce(ctx->grid, ctx->count, ctx->tile_sum, ctx->stream); });
+  if (e == hipSuccess)
+    e = timed(ctx, "bin_scan",
+              [&] { return launch_bin_scan(ctx->grid, ctx->count, ctx->tile_sum, ctx->start, ctx->stream); });
+  if (e == hipSuccess)
+    e = timed(ctx, "bin_scatter", [&] {
+      return launch_bin_scatter(ctx->form, ctx->d, ctx->grid, ctx->slot, ctx->start, ctx->rec, ctx->sidx,
+                                ctx->stream);
+    });
+  if (e == hipSuccess)
+    e = timed(ctx, "pair_scan", [&] {
+      return launch_pair_scan(ctx->form, digest, ctx->d, ctx->grid, fp, ctx->rec, ctx->sidx, ctx->start, ctx->log,
+                              ctx->log_cap, ctx->log_count, ctx->counters, iteration, ctx->stream);
+    });
+  if (e != hipSuccess) return ctx->hipfail(e, "detection launch");
+  return LADDS_OK;
+}
+
+int compact_if_needed(ladds_ctx *ctx) {
+  if (ctx->n == 0) return LADDS_OK;
+  const uint32_t ntiles = (uint32_t)((ctx->n + kCompactTile - 1) / kCompactTile);
+  if (!ctx->d_alt.rx) HIPCHK(ctx, alloc_objects(ctx->d_alt, ctx->cap));
+  DeviceObjects dst = ctx->d_alt;
+  dst.n = ctx->n;
+  hipError_t e = launch_count_alive(ctx->d, ctx->tile_sum, ntiles, ctx->stream);
+  if (e == hipSuccess)
+    e = timed(ctx, "compact",
+              [&] { return launch_compact(ctx->d, dst, ctx->tile_sum, ntiles, ctx->n_alive, ctx->stream); });
+  if (e != hipSuccess) return ctx->hipfail(e, "compaction launch");
+  uint32_t alive = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&alive, ctx->n_alive, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (alive != ctx->n) {
+    std::swap(ctx->d, ctx->d_alt);
+    ctx->d.n = alive;
+    ctx->d_alt.n = 0;
+    ctx->n = alive;
+  }
+  return LADDS_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *ladds_version(void) { return "ladds_amd 0.1 (gfx950)"; }
+int ladds_abi_version(void) { return LADDS_ABI_VERSION; }
+
+int ladds_device_count(int *count) {
+  if (!count) return LADDS_E_INVALID;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *count = c;
+  return LADDS_OK;
+}
+
+void ladds_config_default(ladds_config *cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->dt = 10.0;                       // cfg/default_cfg.yaml:12
+  cfg->t0 = 0.0;                        // Simulation.cpp:143
+  cfg->components = LADDS_KEP;
+  cfg->precision = LADDS_FP64;
+  cfg->cutoff_km = 80.0;                // cfg/default_cfg.yaml:59
+  cfg->collision_distance_factor = 1.0; // Simulation.cpp:233
+  cfg->min_detection_radius_m = 0.05;   // Simulation.cpp:245
+  cfg->evasion_cutoff_km = 0.1;         // Simulation.cpp:246
+  cfg->threshold_mode = LADDS_THRESHOLD_RADIUS_SCALED;
+  cfg->threshold_km = 1.0;
+  cfg->cell_size_km = 0.0;
+  cfg->scan_variant = 0;
+  cfg->block_size = 0;
+  cfg->device = 0;
+  cfg->pair_output = LADDS_PAIRS_LIST;
+  cfg->log_capacity = 1u << 20;
+}
+
+int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
+  if (!cfg || !out) return LADDS_E_INVALID;
+  *out = nullptr;
+  if (!(cfg->dt > 0.0) || !(cfg->cutoff_km > 0.0) || cfg->precision < 0 || cfg->precision > 2 ||
+      cfg->threshold_mode < 0 || cfg->threshold_mode > 1 || cfg->pair_output < 0 || cfg->pair_output > 1)
+    return LADDS_E_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) return LADDS_E_NODEVICE;
+  ladds_ctx *ctx = new ladds_ctx();
+  ctx->cfg = *cfg;
+  ctx->form = cfg->precision;
+  ctx->device = cfg->device;
+  yoshida_coefficients(ctx->c, ctx->dcoef);
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = upload_atmosphere();
+  if (e == hipSuccess) e = dalloc(&ctx->log_count, 1);
+  if (e == hipSuccess) e = dalloc(&ctx->counters, 1);
+  if (e == hipSuccess) e = dalloc(&ctx->n_alive, 1);
+  if (e == hipSuccess) e = hipMemset(ctx->log_count, 0, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(ctx->counters, 0, sizeof(ScanCounters));
+  if (e != hipSuccess) {
+    ladds_destroy(ctx);
+    return (e == hipErrorOutOfMemory) ? LADDS_E_NOMEM : LADDS_E_HIP;
+  }
+  const int rc = ensure_log(ctx, std::max<uint64_t>(cfg->log_capacity, 1024));
+  if (rc != LADDS_OK) {
+    ladds_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return LADDS_OK;
+}
+
+int ladds_destroy(ladds_ctx *ctx) {
+  if (!ctx) return LADDS_E_INVALID;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_objects(ctx->d);
+  free_objects(ctx->d_alt);
+  hipFree(ctx->count);
+  hipFree(ctx->slot);
+  hipFree(ctx->start);
+  hipFree(ctx->tile_sum);
+  hipFree(ctx->sidx);
+  hipFree(ctx->rec);
+  hipFree(ctx->n_alive);
+  hipFree(ctx->env64);
+  hipFree(ctx->env32);
+  hipFree(ctx->log);
+  hipFree(ctx->log_count);
+  hipFree(ctx->counters);
+  for (auto &kv : ctx->prof)
+    for (EventPair &ev : kv.second.used) ctx->event_pool.push_back(ev);
+  for (EventPair &ev : ctx->event_pool) {
+    hipEventDestroy(ev.a);
+    hipEventDestroy(ev.b);
+  }
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return LADDS_OK;
+}
+
+const char *ladds_last_error(const ladds_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
+  if (!ctx || !o) return LADDS_E_INVALID;
+  if (o->n > 0 && (!o->rx || !o->ry || !o->rz || !o->vx || !o->vy || !o->vz || !o->radius_m))
+    return ctx->fail(LADDS_E_INVALID, "positions, velocities and radius_m are required");
+  if (o->n >= (size_t)UINT32_MAX) return ctx->fail(LADDS_E_INVALID, "at most 2^32-2 objects per context");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const size_t n = o->n;
+  if (n > ctx->cap) {
+    free_objects(ctx->d);
+    free_objects(ctx->d_alt);
+    hipFree(ctx->slot);
+    hipFree(ctx->sidx);
+    hipFree(ctx->rec);
+    HIPCHK(ctx, alloc_objects(ctx->d, n));
+    HIPCHK(ctx, dalloc(&ctx->slot, n));
+    HIPCHK(ctx, dalloc(&ctx->sidx, n));
+    HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&ctx->rec), n));
+    ctx->cap = n;
+  }
+  {
+    const int rc = setup_grid(ctx, n);
+    if (rc != LADDS_OK) return rc;
+  }
+  ctx->n = n;
+  ctx->d.n = n;
+  if (n == 0) return LADDS_OK;
+  std::vector<double> radius(o->radius_m, o->radius_m + n), aom(n, 0.0), bc(n, 0.0), cr(n, 1.0), srp64(n),
+      drag64(n);
+  std::vector<float> srp32(n), drag32(n);
+  std::vector<int32_t> act(n, LADDS_PASSIVE);
+  std::vector<uint64_t> id(n), parent(n, UINT64_MAX);
+  for (size_t i = 0; i < n; ++i) {
+    if (o->aom) aom[i] = o->aom[i];
+    if (o->bc_inv) bc[i] = o->bc_inv[i];
+    if (o->cr) cr[i] = o->cr[i];
+    if (o->activity) act[i] = o->activity[i];
+    id[i] = o->id ? o->id[i] : (uint64_t)i;
+    if (o->parent) parent[i] = o->parent[i];
+    if (act[i] < 0 || act[i] > 2) return ctx->fail(LADDS_E_INVALID, "activity must be 0, 1 or 2");
+    srp64[i] = kSrpCoef * cr[i] * aom[i];
+    drag64[i] = kDragCoef * bc[i];
+    srp32[i] = (float)srp64[i];
+    drag32[i] = (float)drag64[i];
+  }
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+  };
+  HIPCHK(ctx, up(ctx->d.rx, o->rx, 8 * n));
+  HIPCHK(ctx, up(ctx->d.ry, o->ry, 8 * n));
+  HIPCHK(ctx, up(ctx->d.rz, o->rz, 8 * n));
+  HIPCHK(ctx, up(ctx->d.vx, o->vx, 8 * n));
+  HIPCHK(ctx, up(ctx->d.vy, o->vy, 8 * n));
+  HIPCHK(ctx, up(ctx->d.vz, o->vz, 8 * n));
+  HIPCHK(ctx, up(ctx->d.radius_m, radius.data(), 8 * n));
+  HIPCHK(ctx, up(ctx->d.aom, aom.data(), 8 * n));
+  HIPCHK(ctx, up(ctx->d.bc_inv, bc.data(), 8 * n));
+  HIPCHK(ctx, up(ctx->d.cr, cr.data(), 8 * n));
+  HIPCHK(ctx, up(ctx->d.srp64, srp64.data(), 8 * n));
+  HIPCHK(ctx, up(ctx->d.drag64, drag64.data(), 8 * n));
+  HIPCHK(ctx, up(ctx->d.srp32, srp32.data(), 4 * n));
+  HIPCHK(ctx, up(ctx->d.drag32, drag32.data(), 4 * n));
+  HIPCHK(ctx, up(ctx->d.activity, act.data(), 4 * n));
+  HIPCHK(ctx, up(ctx->d.id, id.data(), 8 * n));
+  HIPCHK(ctx, up(ctx->d.parent, parent.data(), 8 * n));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return LADDS_OK;
+}
+
+size_t ladds_num_objects(const ladds_ctx *ctx) { return ctx ? ctx->n : 0; }
+
+int ladds_get_state(ladds_ctx *ctx, double *rx, double *ry, double *rz, double *vx, double *vy, double *vz) {
+  if (!ctx) return LADDS_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t b = 8 * ctx->n;
+  double *dst[6] = {rx, ry, rz, vx, vy, vz};
+  double *src[6] = {ctx->d.rx, ctx->d.ry, ctx->d.rz, ctx->d.vx, ctx->d.vy, ctx->d.vz};
+  for (int k = 0; k < 6; ++k)
+    if (dst[k] && b) HIPCHK(ctx, hipMemcpyAsync(dst[k], src[k], b, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return LADDS_OK;
+}
+
+int ladds_get_ids(ladds_ctx *ctx, uint64_t *id) {
+  if (!ctx || !id) return LADDS_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (ctx->n) HIPCHK(ctx, hipMemcpyAsync(id, ctx->d.id, 8 * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return LADDS_OK;
+}
+
+double ladds_get_time(const ladds_ctx *ctx) { return ctx ? ctx->time() : 0.0; }
+
+int ladds_set_time(ladds_ctx *ctx, double t) {
+  if (!ctx) return LADDS_E_INVALID;
+  ctx->cfg.t0 = t;
+  ctx->step_count = 0;
+  return LADDS_OK;
+}
+
+int ladds_propagate(ladds_ctx *ctx, int nsteps) {
+  if (!ctx || nsteps < 0) return LADDS_E_INVALID;
+  if (nsteps == 0 || ctx->n == 0) {
+    ctx->step_count += (uint64_t)nsteps;
+    return LADDS_OK;
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  constexpr int kChunk = 1024;
+  for (int done = 0; done < nsteps;) {
+    const int k = std::min(kChunk, nsteps - done);
+    int rc = ensure_env(ctx, (size_t)k);
+    if (rc == LADDS_OK) rc = launch_propagate_steps(ctx, 0, k, false, 0.0);
+    if (rc != LADDS_OK) return rc;
+    // the host env buffer is reused by the next chunk: wait for the upload
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->step_count += (uint64_t)k;
+    done += k;
+  }
+  return LADDS_OK;
+}
+
+static int run_detection_sync(ladds_ctx *ctx, double dt_interp, bool digest, ScanCounters *out_counters,
+                              uint64_t *entries) {
+  HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
+  int rc = launch_detection(ctx, false, dt_interp, digest, 0);
+  if (rc != LADDS_OK) return rc;
+  unsigned long long cnt = 0;
+  HIPCHK(ctx, hipMemcpyAsync(out_counters, ctx->counters, sizeof(ScanCounters), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->log_count, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *entries = cnt;
+  return LADDS_OK;
+}
+
+int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_t cap, size_t *n, ladds_pair *evaded,
+                       size_t ecap, size_t *ne) {
+  if (!ctx || !n || !ne || (cap && !hits) || (ecap && !evaded)) return LADDS_E_INVALID;
+  *n = *ne = 0;
+  if (ctx->n < 2) return LADDS_OK;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  // the device log is shared with ladds_simulate: preserve its contents
+  unsigned long long saved = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&saved, ctx->log_count, sizeof(saved), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (saved) return ctx->fail(LADDS_E_STATE, "pair log not empty: fetch and clear it before ladds_conjunctions");
+  ScanCounters sc{};
+  uint64_t entries = 0;
+  int rc = run_detection_sync(ctx, dt_interp, false, &sc, &entries);
+  if (rc != LADDS_OK) return rc;
+  if (entries > ctx->log_cap) {
+    rc = ensure_log(ctx, entries);
+    if (rc == LADDS_OK) rc = run_detection_sync(ctx, dt_interp, false, &sc, &entries);
+    if (rc != LADDS_OK) return rc;
+  }
+  std::vector<LoggedPair> h(entries);
+  if (entries)
+    HIPCHK(ctx, hipMemcpyAsync(h.data(), ctx->log, sizeof(LoggedPair) * entries, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  size_t nh = 0, nev = 0;
+  for (const LoggedPair &lp : h) {
+    if (lp.evaded) {
+      if (nev < ecap) std::memcpy(&evaded[nev], &lp.pair, sizeof(ladds_pair));
+      ++nev;
+    } else {
+      if (nh < cap) std::memcpy(&hits[nh], &lp.pair, sizeof(ladds_pair));
+      ++nh;
+    }
+  }
+  *n = nh;
+  *ne = nev;
+  if (nh > cap || nev > ecap) return ctx->fail(LADDS_E_OVERFLOW, "pair output capacity too small");
+  return LADDS_OK;
+}
+
+int ladds_conjunction_digest(ladds_ctx *ctx, double dt_interp, uint64_t *n, uint64_t digest[2], uint64_t *ne,
+                             uint64_t edigest[2]) {
+  if (!ctx || !n || !digest || !ne || !edigest) return LADDS_E_INVALID;
+  *n = *ne = 0;
+  digest[0] = digest[1] = edigest[0] = edigest[1] = 0;
+  if (ctx->n < 2) return LADDS_OK;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  ScanCounters sc{};
+  uint64_t entries = 0;
+  unsigned long long saved = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&saved, ctx->log_count, sizeof(saved), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const int rc = run_detection_sync(ctx, dt_interp, true, &sc, &entries);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->log_count, &saved, sizeof(saved), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (rc != LADDS_OK) return rc;
+  *n = sc.hits;
+  *ne = sc.evaded;
+  digest[0] = sc.digest[0];
+  digest[1] = sc.digest[1];
+  edigest[0] = sc.edigest[0];
+  edigest[1] = sc.edigest[1];
+  return LADDS_OK;
+}
+
+int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpcd, double min_altitude_km,
+                   ladds_run_stats *stats) {
+  if (!ctx || nsteps < 0 || tpcd < 1) return LADDS_E_INVALID;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
+  const bool digest = ctx->cfg.pair_output == LADDS_PAIRS_DIGEST;
+  const double burnup_r = min_altitude_km > 0.0 ? min_altitude_km + kREarth : 0.0;
+  const double burnup_r2 = burnup_r * burnup_r;  // Simulation.cpp:576,583
+  constexpr int kChunk = 1024;
+  int done = 0;
+  while (done < nsteps && ctx->n > 0) {
+    const int chunk = std::min(kChunk, nsteps - done);
+    int rc = ensure_env(ctx, (size_t)chunk);
+    if (rc != LADDS_OK) return rc;
+    int s = 0;
+    while (s < chunk) {
+      // integrate iterations up to (and including) the next detection iteration
+      int k = 1;
+      uint64_t it = first_iteration + (uint64_t)(done + s);
+      while (s + k < chunk && it % (uint64_t)tpcd != 0) {
+        ++k;
+        ++it;
+      }
+      const bool detect = (it % (uint64_t)tpcd) == 0;
+      rc = launch_propagate_steps(ctx, (size_t)s, k, detect, burnup_r2);
+      if (rc == LADDS_OK && detect)
+        rc = launch_detection(ctx, true, ctx->cfg.dt * (double)tpcd, digest, it);
+      if (rc != LADDS_OK) return rc;
+      s += k;
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->step_count += (uint64_t)chunk;
+    done += chunk;
+  }
+  ScanCounters sc{};
+  unsigned long long cnt = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&sc, ctx->counters, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->log_count, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (min_altitude_km > 0.0) {
+    const int rc = compact_if_needed(ctx);
+    if (rc != LADDS_OK) return rc;
+  }
+  if (stats) {
+    stats->collisions = sc.hits;
+    stats->evasions = sc.evaded;
+    stats->logged = std::min<uint64_t>(cnt, ctx->log_cap);
+    stats->candidates = sc.candidates;
+    stats->digest[0] = sc.digest[0];
+    stats->digest[1] = sc.digest[1];
+    stats->edigest[0] = sc.edigest[0];
+    stats->edigest[1] = sc.edigest[1];
+    stats->remaining = ctx->n;
+  }
+  return LADDS_OK;
+}
+
+int ladds_fetch_log(ladds_ctx *ctx, ladds_logged_pair *out, size_t cap, size_t *n) {
+  if (!ctx || !n || (cap && !out)) return LADDS_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  unsigned long long cnt = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->log_count, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const uint64_t stored = std::min<uint64_t>(cnt, ctx->log_cap);
+  *n = (size_t)cnt;
+  const size_t m = std::min<size_t>(stored, cap);
+  if (m) HIPCHK(ctx, hipMemcpyAsync(out, ctx->log, sizeof(LoggedPair) * m, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (cnt > ctx->log_cap) return ctx->fail(LADDS_E_OVERFLOW, "device pair log overflowed (log_capacity)");
+  if (cnt > cap) return ctx->fail(LADDS_E_OVERFLOW, "output capacity too small");
+  return LADDS_OK;
+}
+
+int ladds_clear_log(ladds_ctx *ctx) {
+  if (!ctx) return LADDS_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return LADDS_OK;
+}
+
+int ladds_delete_burnups(ladds_ctx *ctx, double min_altitude_km, size_t *removed) {
+  if (!ctx) return LADDS_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t before = ctx->n;
+  if (before) {
+    // mark, then compact: a zero-step propagation applies only the burn-up test
+    const double r = min_altitude_km + kREarth;
+    std::vector<double> hx(before), hy(before), hz(before);
+    std::vector<int32_t> act(before);
+    HIPCHK(ctx, hipMemcpyAsync(hx.data(), ctx->d.rx, 8 * before, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(hy.data(), ctx->d.ry, 8 * before, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(hz.data(), ctx->d.rz, 8 * before, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(act.data(), ctx->d.activity, 4 * before, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const double r2 = r * r;
+    for (size_t i = 0; i < before; ++i) {
+      double s = 0.0;
+      s += hx[i] * hx[i];
+      s += hy[i] * hy[i];
+      s += hz[i] * hz[i];
+      if (s < r2) act[i] = -1;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d.activity, act.data(), 4 * before, hipMemcpyHostToDevice, ctx->stream));
+    const int rc = compact_if_needed(ctx);
+    if (rc != LADDS_OK) return rc;
+  }
+  if (removed) *removed = before - ctx->n;
+  return LADDS_OK;
+}
+
+int ladds_set_profiling(ladds_ctx *ctx, int enable) {
+  if (!ctx) return LADDS_E_INVALID;
+  resolve_profiles(ctx);
+  ctx->profiling = enable != 0;
+  for (auto &kv : ctx->prof) {
+    kv.second.launches = 0;
+    kv.second.ms = 0.0;
+  }
+  return LADDS_OK;
+}
+
+int ladds_kernel_stats(ladds_ctx *ctx, const char *name, uint64_t *launches, double *total_ms) {
+  if (!ctx || !name || !launches || !total_ms) return LADDS_E_INVALID;
+  bool known = false;
+  for (const char *k : kKernelNames) known |= std::strcmp(k, name) == 0;
+  if (!known) return ctx->fail(LADDS_E_INVALID, std::string("unknown kernel name ") + name);
+  resolve_profiles(ctx);
+  auto it = ctx->prof.find(name);
+  *launches = it == ctx->prof.end() ? 0 : it->second.launches;
+  *total_ms = it == ctx->prof.end() ? 0.0 : it->second.ms;
+  return LADDS_OK;
+}
+
+int ladds_synchronize(ladds_ctx *ctx) {
+  if (!ctx) return LADDS_E_INVALID;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return LADDS_OK;
+}
+
+void *ladds_stream(ladds_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+}  // extern "C"

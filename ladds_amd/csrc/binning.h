@@ -1,0 +1,54 @@
+// binning.h -- cell assignment shared by the propagation (count pass) and the scatter.
+//
+// Cells have side s >= 2 * cutoff.  An object at fractional cell position f < 1/2
+// (per axis) can only have partners within the cutoff in its own cell or the one
+// below, otherwise in its own cell or the one above; its search block is the
+// 2x2x2 cells starting at (ix - [f < 1/2], ...).  Any pair within the cutoff lies
+// in each other's search block, so scanning the 8 buckets of the block from the
+// lower sorted position finds every pair exactly once.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+
+namespace ladds {
+
+// autopas::utils::ArrayMath::dot: accumulate from 0, no contraction: ((0 + a0 b0) + a1 b1) + a2 b2.
+__device__ __forceinline__ double exact_dot3(double a0, double a1, double a2, double b0, double b1, double b2) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+  s += a0 * b0;
+  s += a1 * b1;
+  s += a2 * b2;
+  return s;
+}
+
+struct CellInfo {
+  uint32_t bucket;  // folded bucket of the object's own cell
+  uint32_t base;    // folded lower corner of its 2x2x2 search block, packed (fx, fy, fz)
+};
+
+__device__ __forceinline__ int32_t cell_axis(double x, double inv_cell, uint32_t &low) {
+  double u = x * inv_cell;
+  u = fmin(fmax(u, -1.0e9), 1.0e9);
+  const double fl = floor(u);
+  low = (u - fl < 0.5) ? 1u : 0u;
+  return (int32_t)fl;
+}
+
+__device__ __forceinline__ CellInfo cell_of(double x, double y, double z, const GridParams &g) {
+  uint32_t lx, ly, lz;
+  const int32_t ix = cell_axis(x, g.inv_cell, lx);
+  const int32_t iy = cell_axis(y, g.inv_cell, ly);
+  const int32_t iz = cell_axis(z, g.inv_cell, lz);
+  const uint32_t mx = (1u << g.bx) - 1u, my = (1u << g.by) - 1u, mz = (1u << g.bz) - 1u;
+  const uint32_t sy = g.bz, sx = g.by + g.bz;
+  CellInfo c;
+  c.bucket = (((uint32_t)ix & mx) << sx) | (((uint32_t)iy & my) << sy) | ((uint32_t)iz & mz);
+  c.base = (((uint32_t)(ix - (int32_t)lx) & mx) << sx) | (((uint32_t)(iy - (int32_t)ly) & my) << sy) |
+           ((uint32_t)(iz - (int32_t)lz) & mz);
+  return c;
+}
+
+}  // namespace ladds

@@ -1,0 +1,97 @@
+// engine.h -- internal types shared by the host runtime and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace ladds {
+
+// Precision forms of the propagation (values of ladds_config.precision).
+enum : int { kFormFp64 = 0, kFormSplit = 1, kFormSplitAll32 = 2 };
+
+// Uniform environment of one Yoshida sub-step (identical for every object):
+// Earth rotation angle and Sun/Moon positions, precomputed on the host in FP64.
+// The third-body terms use Battin's form a = -GM/|s|^3 (1+q)^-3/2 (r + f(q) s)
+// (Battin 1987, eq. 8.61) which has no cancellation, so FP32 evaluation keeps
+// the relative accuracy of the tidal term itself.
+template <typename T>
+struct SubstepEnv {
+  T ct, st;               // cos/sin of the Earth rotation angle
+  T sun[3];               // Sun position [km]
+  T sun_inv_s2;           // 1 / |s|^2
+  T sun_k;                // GM_sun / |s|^3
+  T moon[3];              // Moon position [km]
+  T moon_inv_s2;          // 1 / |m|^2
+  T moon_k;               // GM_moon / |m|^3
+  T pad[4];
+};
+static_assert(sizeof(SubstepEnv<double>) == 16 * 8, "env layout");
+static_assert(sizeof(SubstepEnv<float>) == 16 * 4, "env layout");
+
+// Folded spatial grid: cell (ix, iy, iz) -> bucket ((ix & mx) * Ky + (iy & my)) * Kz + (iz & mz).
+// Neighbouring cells stay adjacent in bucket order (z fastest), so buckets hold
+// spatially coherent runs; cells that alias into one bucket are at least
+// K * cell_size apart and are rejected by the distance test.
+struct GridParams {
+  double inv_cell;       // 1 / cell size [1/km]
+  uint32_t bx, by, bz;   // log2 of Kx, Ky, Kz
+  uint32_t nbuckets;     // Kx * Ky * Kz
+};
+
+// FP64 sorted record (plain FP64 form) and FP32 record (split form).  `base`
+// packs the folded lower corner (fx, fy, fz) of the object's 2x2x2 search block.
+struct alignas(32) RecordF64 {
+  double x, y, z;
+  uint32_t base;
+  uint32_t pad;
+};
+struct alignas(16) RecordF32 {
+  float x, y, z;
+  uint32_t base;
+};
+
+// Device view of the object arrays.
+struct DeviceObjects {
+  size_t n;
+  double *rx, *ry, *rz, *vx, *vy, *vz;
+  double *radius_m, *aom, *bc_inv, *cr;
+  double *srp64, *drag64;   // kSrpCoef * cr * aom, kDragCoef * bc_inv
+  float *srp32, *drag32;
+  int32_t *activity;
+  uint64_t *id, *parent;
+};
+
+struct PairRecord {  // same layout as ladds_pair
+  uint64_t id_a, id_b;
+  uint32_t idx_a, idx_b;
+  double d2;
+  double mid[3];
+};
+
+struct LoggedPair {  // same layout as ladds_logged_pair
+  PairRecord pair;
+  uint64_t iteration;
+  uint32_t evaded;
+  uint32_t reserved;
+};
+
+// Counters written by the pair scan (device memory).
+struct ScanCounters {
+  unsigned long long hits, evaded, candidates, pad;
+  unsigned long long digest[2], edigest[2];
+};
+
+struct FunctorParams {
+  double cutoff2;         // cutoff^2 [km^2]
+  double cdf_km;          // collisionDistanceFactor / 1000
+  double evasion2;        // evasionTrackingCutoff^2 [km^2]
+  double min_det_radius;  // [m]
+  double threshold2;      // absolute mode threshold^2
+  double dt;              // interpolation window [s]
+  int threshold_mode;
+  float cutoff2_f32;      // inflated FP32 prefilter radius^2 (split form)
+};
+
+}  // namespace ladds

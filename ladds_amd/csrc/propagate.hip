@@ -1,0 +1,137 @@
+// propagate.hip -- Yoshida-4 propagation kernel (one thread per object).
+//
+// Replaces YoshidaIntegrator<AutoPas_t>::integrate (OrbitPropagator @ b90f3bae, called at
+// src/ladds/Simulation.cpp:297).  One step is the 4th-order composition of Yoshida
+// (Phys. Lett. A 150 (1990) 262):
+//   r += c1 dt v; v += d1 dt a(r, v, t + c1 dt)
+//   r += c2 dt v; v += d2 dt a(r, v, t + (c1+c2) dt)
+//   r += c3 dt v; v += d3 dt a(r, v, t + (c1+c2+c3) dt)
+//   r += c4 dt v
+// with w1 = 1/(2 - 2^(1/3)), w0 = -2^(1/3)/(2 - 2^(1/3)), c1 = c4 = w1/2,
+// c2 = c3 = (w0+w1)/2, d1 = d3 = w1, d2 = w0.  Velocity-dependent drag uses the
+// latest kicked velocity; time-dependent terms use the sub-step time.
+//
+// Layout: SoA FP64 state in HBM, one coalesced 8-byte load/store per component.
+// Several steps per launch keep the state in registers between steps (used when
+// no conjunction scan is due in between).  When `count` is set, the kernel also
+// performs the counting pass of the cell binning for the final positions.
+#include "binning.h"
+#include "launch.h"
+#include "physics.h"
+
+namespace ladds {
+
+__constant__ double c_atm_h0[kAtmRows];
+__constant__ double c_atm_rho[kAtmRows];
+__constant__ double c_atm_inv_scale[kAtmRows];
+
+struct PropagateArgs {
+  DeviceObjects o;
+  const SubstepEnv<double> *env64;  // 3 per step
+  const SubstepEnv<float> *env32;
+  int nsteps;
+  unsigned mask;
+  double cdt[4], ddt[3];
+  GridParams grid;
+  uint32_t *count;  // non-null: binning count pass
+  uint32_t *slot;
+  double burnup_r2;  // > 0: objects with |r|^2 below become dummies (deleteBurnUps)
+};
+
+template <int FORM>
+__global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.o.n) return;
+  double x = a.o.rx[i], y = a.o.ry[i], z = a.o.rz[i];
+  double vx = a.o.vx[i], vy = a.o.vy[i], vz = a.o.vz[i];
+  ObjParams p;
+  p.activity = a.o.activity[i];
+  // dummies (burnt-up objects awaiting compaction) keep their state
+  const int nsteps = p.activity < 0 ? 0 : a.nsteps;
+  if constexpr (FORM == kFormFp64) {
+    p.srp64 = a.o.srp64[i];
+    p.drag64 = a.o.drag64[i];
+    p.srp32 = p.drag32 = 0.f;
+  } else {
+    p.srp32 = a.o.srp32[i];
+    p.drag32 = a.o.drag32[i];
+    p.srp64 = p.drag64 = 0.0;
+  }
+  bool burnt = false;
+  for (int s = 0; s < nsteps; ++s) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double cdt = a.cdt[k], ddt = a.ddt[k];
+      x += cdt * vx;
+      y += cdt * vy;
+      z += cdt * vz;
+      double ax, ay, az;
+      acceleration<FORM>(x, y, z, vx, vy, vz, p, a.mask, a.env64[3 * s + k], a.env32[3 * s + k], ax, ay, az);
+      vx += ddt * ax;
+      vy += ddt * ay;
+      vz += ddt * az;
+    }
+    x += a.cdt[3] * vx;
+    y += a.cdt[3] * vy;
+    z += a.cdt[3] * vz;
+    // Simulation::deleteBurnUps after every integrate (Simulation.cpp:301-303, 589-590)
+    if (a.burnup_r2 > 0.0 && exact_dot3(x, y, z, x, y, z) < a.burnup_r2) {
+      burnt = true;
+      break;
+    }
+  }
+  if (burnt) a.o.activity[i] = -1;
+  a.o.rx[i] = x;
+  a.o.ry[i] = y;
+  a.o.rz[i] = z;
+  a.o.vx[i] = vx;
+  a.o.vy[i] = vy;
+  a.o.vz[i] = vz;
+  if (a.count) {
+    const CellInfo c = cell_of(x, y, z, a.grid);
+    a.slot[i] = atomicAdd(&a.count[c.bucket], 1u);
+  }
+}
+
+hipError_t upload_atmosphere() {
+  double inv[kAtmRows];
+  for (int i = 0; i < kAtmRows; ++i) inv[i] = 1.0 / kAtmScale[i];
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_atm_h0), kAtmH0, sizeof(kAtmH0));
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_atm_rho), kAtmRho, sizeof(kAtmRho));
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_atm_inv_scale), inv, sizeof(inv));
+  return e;
+}
+
+hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<double> *env64,
+                            const SubstepEnv<float> *env32, int nsteps, unsigned mask, const double cdt[4],
+                            const double ddt[3], const GridParams *grid, uint32_t *count, uint32_t *slot,
+                            double burnup_r2, hipStream_t stream) {
+  if (o.n == 0 || nsteps <= 0) return hipSuccess;
+  PropagateArgs a;
+  a.o = o;
+  a.env64 = env64;
+  a.env32 = env32;
+  a.nsteps = nsteps;
+  a.mask = mask;
+  for (int k = 0; k < 4; ++k) a.cdt[k] = cdt[k];
+  for (int k = 0; k < 3; ++k) a.ddt[k] = ddt[k];
+  if (grid) a.grid = *grid;
+  a.count = count;
+  a.slot = slot;
+  a.burnup_r2 = burnup_r2;
+  const unsigned blocks = (unsigned)((o.n + 255) / 256);
+  switch (form) {
+    case kFormFp64:
+      hipLaunchKernelGGL(k_propagate<kFormFp64>, dim3(blocks), dim3(256), 0, stream, a);
+      break;
+    case kFormSplit:
+      hipLaunchKernelGGL(k_propagate<kFormSplit>, dim3(blocks), dim3(256), 0, stream, a);
+      break;
+    default:
+      hipLaunchKernelGGL(k_propagate<kFormSplitAll32>, dim3(blocks), dim3(256), 0, stream, a);
+      break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace ladds

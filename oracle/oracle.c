@@ -1,0 +1,569 @@
+/*
+ * oracle.c -- CPU restatement of the LADDS propagate + conjunction hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  See oracle.h
+ * for what each function follows and what pins it.  Build: oracle/Makefile
+ * (gcc -O2 -ffp-contract=off -fno-fast-math -fopenmp).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* constants                                                                  */
+/* ------------------------------------------------------------------------- */
+/* Earth gravity field: EGM96 (GM, reference radius, unnormalised C20/C22/S22
+ * obtained from the normalised coefficients C20=-4.84165371736e-4,
+ * C22=2.43914352398e-6, S22=-1.40016683654e-6).  GM = 398600.4418 km^3/s^2 is
+ * also the value the reference's 1-s Kepler vector (SimulationTest.cpp:39-57)
+ * reproduces. */
+#define MU_EARTH 398600.4418
+#define R_EARTH 6378.1363
+#define J2_EARTH 1.0826266835531513e-3
+#define C22_EARTH 1.574460374564035e-6
+#define S22_EARTH -9.03803806638557e-7
+/* third bodies (DE405 GM values) and the astronomical unit (IAU 2012) */
+#define GM_SUN 1.32712440018e11
+#define GM_MOON 4902.800066
+#define AU_KM 149597870.7
+/* solar radiation pressure at 1 AU [N/m^2], Montenbruck & Gill eq. 3.69 */
+#define P_SUN 4.56e-6
+/* Earth rotation rate [rad/s] and GMST at J2000.0 (280.46061837504 deg) */
+#define OMEGA_EARTH 7.2921158553e-5
+#define THETA_J2000 4.894961212823756
+/* obliquity of the ecliptic at J2000 (IAU 1976: 84381.448 arcsec) */
+#define OBLIQUITY 0.40909280422232897
+#define SEC_PER_CENTURY 3155760000.0
+#define DEG (3.14159265358979323846 / 180.0)
+#define ARCSEC (DEG / 3600.0)
+#define TWO_PI 6.28318530717958647692
+
+double or_constant(const char *name) {
+  static const struct {
+    const char *n;
+    double v;
+  } tab[] = {{"mu", MU_EARTH},       {"r_earth", R_EARTH},     {"j2", J2_EARTH},
+             {"c22", C22_EARTH},     {"s22", S22_EARTH},       {"gm_sun", GM_SUN},
+             {"gm_moon", GM_MOON},   {"au", AU_KM},            {"p_sun", P_SUN},
+             {"omega_earth", OMEGA_EARTH}, {"theta_j2000", THETA_J2000}, {"obliquity", OBLIQUITY}};
+  for (size_t i = 0; i < sizeof(tab) / sizeof(tab[0]); ++i)
+    if (strcmp(tab[i].n, name) == 0) return tab[i].v;
+  return NAN;
+}
+
+/* Vallado (2013) Table 8-4: base altitude [km], nominal density [kg/m^3], scale height [km] */
+static const double ATM_H0[28] = {0,   25,  30,  40,  50,  60,  70,  80,  90,  100, 110, 120, 130, 140,
+                                  150, 180, 200, 250, 300, 350, 400, 450, 500, 600, 700, 800, 900, 1000};
+static const double ATM_RHO[28] = {1.225,     3.899e-2,  1.774e-2,  3.972e-3,  1.057e-3,  3.206e-4,  8.770e-5,
+                                   1.905e-5,  3.396e-6,  5.297e-7,  9.661e-8,  2.438e-8,  8.484e-9,  3.845e-9,
+                                   2.070e-9,  5.464e-10, 2.789e-10, 7.248e-11, 2.418e-11, 9.518e-12, 3.725e-12,
+                                   1.585e-12, 6.967e-13, 1.454e-13, 3.614e-14, 1.170e-14, 5.245e-15, 3.019e-15};
+static const double ATM_H[28] = {7.249,  6.349,  6.682,  7.554,  8.382,  7.714,  6.549,  5.799,  5.382,  5.877,
+                                 7.263,  9.473,  12.636, 16.149, 22.523, 29.740, 37.105, 45.546, 53.628, 53.298,
+                                 58.515, 60.828, 63.822, 71.835, 88.667, 124.64, 181.05, 268.00};
+
+double or_density(double h) {
+  int i = 0;
+  /* largest i with h0[i] <= h; below 0 km the first row is used */
+  for (int k = 27; k >= 0; --k) {
+    if (h >= ATM_H0[k]) {
+      i = k;
+      break;
+    }
+  }
+  return ATM_RHO[i] * exp(-(h - ATM_H0[i]) / ATM_H[i]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* ephemerides                                                                */
+/* ------------------------------------------------------------------------- */
+void or_sun_position(double t, double out[3]) {
+  /* Montenbruck & Gill (2000) eq. 3.46-3.47: mean anomaly, ecliptic longitude
+   * (Omega+omega = 282.9400 deg), distance; ecliptic latitude 0. */
+  const double T = t / SEC_PER_CENTURY;
+  const double M = (357.5256 + 35999.049 * T) * DEG;
+  const double L = 282.9400 * DEG + M + (6892.0 * sin(M) + 72.0 * sin(2.0 * M)) * ARCSEC;
+  const double r = (149.619 - 2.499 * cos(M) - 0.021 * cos(2.0 * M)) * 1.0e6;
+  const double ce = cos(OBLIQUITY), se = sin(OBLIQUITY);
+  out[0] = r * cos(L);
+  out[1] = r * sin(L) * ce;
+  out[2] = r * sin(L) * se;
+}
+
+void or_moon_position(double t, double out[3]) {
+  /* Montenbruck & Gill (2000) eq. 3.48-3.50: low-precision lunar series. */
+  const double T = t / SEC_PER_CENTURY;
+  const double L0 = (218.31617 + 481267.88088 * T - 1.3972 * T) * DEG;
+  const double l = (134.96292 + 477198.86753 * T) * DEG;
+  const double lp = (357.52543 + 35999.04944 * T) * DEG;
+  const double F = (93.27283 + 483202.01873 * T) * DEG;
+  const double D = (297.85027 + 445267.11135 * T) * DEG;
+  const double lam =
+      L0 + (22640.0 * sin(l) + 769.0 * sin(2.0 * l) - 4586.0 * sin(l - 2.0 * D) + 2370.0 * sin(2.0 * D) -
+            668.0 * sin(lp) - 412.0 * sin(2.0 * F) - 212.0 * sin(2.0 * l - 2.0 * D) - 206.0 * sin(l + lp - 2.0 * D) +
+            192.0 * sin(l + 2.0 * D) - 165.0 * sin(lp - 2.0 * D) + 148.0 * sin(l - lp) - 125.0 * sin(D) -
+            110.0 * sin(l + lp) - 55.0 * sin(2.0 * F - 2.0 * D)) *
+               ARCSEC;
+  const double beta =
+      (18520.0 * sin(F + lam - L0 + (412.0 * sin(2.0 * F) + 541.0 * sin(lp)) * ARCSEC) - 526.0 * sin(F - 2.0 * D) +
+       44.0 * sin(l + F - 2.0 * D) - 31.0 * sin(-l + F - 2.0 * D) - 25.0 * sin(-2.0 * l + F) -
+       23.0 * sin(lp + F - 2.0 * D) + 21.0 * sin(-l + F) + 11.0 * sin(-lp + F - 2.0 * D)) *
+      ARCSEC;
+  const double r = 385000.0 - 20905.0 * cos(l) - 3699.0 * cos(2.0 * D - l) - 2956.0 * cos(2.0 * D) -
+                   570.0 * cos(2.0 * l) + 246.0 * cos(2.0 * l - 2.0 * D) - 205.0 * cos(lp - 2.0 * D) -
+                   171.0 * cos(l + 2.0 * D) - 152.0 * cos(l + lp - 2.0 * D);
+  const double xe = r * cos(lam) * cos(beta);
+  const double ye = r * sin(lam) * cos(beta);
+  const double ze = r * sin(beta);
+  const double ce = cos(OBLIQUITY), se = sin(OBLIQUITY);
+  out[0] = xe;
+  out[1] = ye * ce - ze * se;
+  out[2] = ye * se + ze * ce;
+}
+
+double or_earth_rotation_angle(double t) {
+  double th = fmod(THETA_J2000 + OMEGA_EARTH * t, TWO_PI);
+  if (th < 0.0) th += TWO_PI;
+  return th;
+}
+
+/* ------------------------------------------------------------------------- */
+/* force model                                                                */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  double ct, st;                /* cos/sin Earth rotation angle */
+  double sun[3], moon[3];       /* positions [km] */
+} or_env;
+
+static void env_at(double t, or_env *e) {
+  const double th = or_earth_rotation_angle(t);
+  e->ct = cos(th);
+  e->st = sin(th);
+  or_sun_position(t, e->sun);
+  or_moon_position(t, e->moon);
+}
+
+static void third_body(const double r[3], const double s[3], double gm, double a[3]) {
+  /* Montenbruck & Gill eq. 3.37 */
+  const double d[3] = {s[0] - r[0], s[1] - r[1], s[2] - r[2]};
+  const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const double dn = sqrt(d2);
+  const double s2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+  const double sn = sqrt(s2);
+  const double fd = gm / (d2 * dn);
+  const double fs = gm / (s2 * sn);
+  a[0] += fd * d[0] - fs * s[0];
+  a[1] += fd * d[1] - fs * s[1];
+  a[2] += fd * d[2] - fs * s[2];
+}
+
+static void accel_env(const double r[3], const double v[3], unsigned mask, const or_params *p, const or_env *e,
+                      double a[3]) {
+  a[0] = a[1] = a[2] = 0.0;
+  if (p->activity == OR_EVASIVE_PRESERVING) mask &= OR_KEP; /* Particle.h:49-52 */
+  const double x = r[0], y = r[1], z = r[2];
+  const double r2 = x * x + y * y + z * z;
+  const double rn = sqrt(r2);
+  if (mask & OR_KEP) {
+    const double f = -MU_EARTH / (r2 * rn);
+    a[0] += f * x;
+    a[1] += f * y;
+    a[2] += f * z;
+  }
+  if (mask & OR_J2) {
+    /* gradient of -J2 mu R^2 / (2 r^3) (3 z^2/r^2 - 1) (M&G eq. 3.33 with n=2, m=0) */
+    const double z2r2 = z * z / r2;
+    const double f = -(1.5 * J2_EARTH * MU_EARTH * R_EARTH * R_EARTH) / (r2 * r2 * rn);
+    a[0] += f * x * (1.0 - 5.0 * z2r2);
+    a[1] += f * y * (1.0 - 5.0 * z2r2);
+    a[2] += f * z * (3.0 - 5.0 * z2r2);
+  }
+  if (mask & (OR_C22 | OR_S22)) {
+    /* degree-2 order-2 tesseral terms in the Earth-fixed frame (rotation about z by theta):
+     * U22 = 3 mu R^2 (C22 (x^2 - y^2) + 2 S22 x y) / r^5  (M&G eq. 3.8 with P22 = 3 cos^2 phi) */
+    const double xe = e->ct * x + e->st * y;
+    const double ye = -e->st * x + e->ct * y;
+    const double ze = z;
+    const double inv_r5 = 1.0 / (r2 * r2 * rn);
+    const double inv_r7 = inv_r5 / r2;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (mask & OR_C22) {
+      const double k = 3.0 * MU_EARTH * R_EARTH * R_EARTH * C22_EARTH;
+      const double q = xe * xe - ye * ye;
+      ax += k * (2.0 * xe * inv_r5 - 5.0 * xe * q * inv_r7);
+      ay += k * (-2.0 * ye * inv_r5 - 5.0 * ye * q * inv_r7);
+      az += k * (-5.0 * ze * q * inv_r7);
+    }
+    if (mask & OR_S22) {
+      const double k = 6.0 * MU_EARTH * R_EARTH * R_EARTH * S22_EARTH;
+      const double q = xe * ye;
+      ax += k * (ye * inv_r5 - 5.0 * xe * q * inv_r7);
+      ay += k * (xe * inv_r5 - 5.0 * ye * q * inv_r7);
+      az += k * (-5.0 * ze * q * inv_r7);
+    }
+    a[0] += e->ct * ax - e->st * ay;
+    a[1] += e->st * ax + e->ct * ay;
+    a[2] += az;
+  }
+  if (mask & OR_SOL) third_body(r, e->sun, GM_SUN, a);
+  if (mask & OR_LUN) third_body(r, e->moon, GM_MOON, a);
+  if (mask & OR_SRP) {
+    /* cannonball SRP, M&G eq. 3.75, with the cylindrical shadow model (M&G Sec. 3.4.2) */
+    const double *s = e->sun;
+    const double d[3] = {x - s[0], y - s[1], z - s[2]};
+    const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double dn = sqrt(d2);
+    const double s2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+    const double sr = s[0] * x + s[1] * y + s[2] * z;
+    int lit = 1;
+    if (sr < 0.0) {
+      const double perp2 = r2 - sr * sr / s2;
+      if (perp2 < R_EARTH * R_EARTH) lit = 0;
+    }
+    if (lit) {
+      const double f = P_SUN * 1.0e3 * AU_KM * AU_KM * p->cr * p->aom / (d2 * dn);
+      a[0] += f * d[0];
+      a[1] += f * d[1];
+      a[2] += f * d[2];
+    }
+  }
+  if (mask & OR_DRAG) {
+    /* M&G eq. 3.97 with co-rotating atmosphere: v_rel = v - omega x r */
+    const double vr[3] = {v[0] + OMEGA_EARTH * y, v[1] - OMEGA_EARTH * x, v[2]};
+    const double vn = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+    const double rho = or_density(rn - R_EARTH);
+    const double f = -0.5e3 * p->bc_inv * rho * vn;
+    a[0] += f * vr[0];
+    a[1] += f * vr[1];
+    a[2] += f * vr[2];
+  }
+}
+
+void or_acceleration(const double r[3], const double v[3], double t, unsigned mask, const or_params *p,
+                     double a[3]) {
+  or_env e;
+  env_at(t, &e);
+  accel_env(r, v, mask, p, &e, a);
+}
+
+void or_yoshida_coefficients(double c[4], double d[3]) {
+  const double cb = cbrt(2.0);
+  const double w1 = 1.0 / (2.0 - cb);
+  const double w0 = -cb / (2.0 - cb);
+  c[0] = c[3] = w1 / 2.0;
+  c[1] = c[2] = (w0 + w1) / 2.0;
+  d[0] = d[2] = w1;
+  d[1] = w0;
+}
+
+void or_yoshida_steps(size_t n, double *rx, double *ry, double *rz, double *vx, double *vy, double *vz,
+                      const double *aom, const double *bc_inv, const double *cr, const int32_t *activity,
+                      double t0, double dt, unsigned mask, int nsteps, int nthreads) {
+  double c[4], d[3];
+  or_yoshida_coefficients(c, d);
+  const double tau[3] = {c[0], c[0] + c[1], c[0] + c[1] + c[2]};
+  or_env *env = (or_env *)malloc(sizeof(or_env) * 3 * (size_t)(nsteps > 0 ? nsteps : 1));
+  for (int s = 0; s < nsteps; ++s) {
+    const double t = t0 + (double)s * dt;
+    for (int k = 0; k < 3; ++k) env_at(t + tau[k] * dt, &env[3 * s + k]);
+  }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+#endif
+  for (long long i = 0; i < (long long)n; ++i) {
+    or_params p = {aom ? aom[i] : 0.0, bc_inv ? bc_inv[i] : 0.0, cr ? cr[i] : 1.0, activity ? activity[i] : 0};
+    double r[3] = {rx[i], ry[i], rz[i]}, v[3] = {vx[i], vy[i], vz[i]}, a[3];
+    for (int s = 0; s < nsteps; ++s) {
+      for (int k = 0; k < 3; ++k) {
+        const double cdt = c[k] * dt, ddt = d[k] * dt;
+        r[0] += cdt * v[0];
+        r[1] += cdt * v[1];
+        r[2] += cdt * v[2];
+        accel_env(r, v, mask, &p, &env[3 * s + k], a);
+        v[0] += ddt * a[0];
+        v[1] += ddt * a[1];
+        v[2] += ddt * a[2];
+      }
+      const double cdt = c[3] * dt;
+      r[0] += cdt * v[0];
+      r[1] += cdt * v[1];
+      r[2] += cdt * v[2];
+    }
+    rx[i] = r[0];
+    ry[i] = r[1];
+    rz[i] = r[2];
+    vx[i] = v[0];
+    vy[i] = v[1];
+    vz[i] = v[2];
+  }
+  free(env);
+}
+
+/* ------------------------------------------------------------------------- */
+/* conjunction predicate                                                      */
+/* ------------------------------------------------------------------------- */
+/* autopas::utils::ArrayMath::dot: result = 0; result += a[k] * b[k] for k = 0..2 */
+static inline double dot3(const double a[3], const double b[3]) {
+  double s = 0.0;
+  s += a[0] * b[0];
+  s += a[1] * b[1];
+  s += a[2] * b[2];
+  return s;
+}
+
+void or_distance_by_linear_interpolation(const double ri[3], const double vi[3], const double rj[3],
+                                         const double vj[3], double dt, double dr[3], double p2[3]) {
+  /* DistanceApproximation.cpp:25-55, operation for operation */
+  double oi[3], oj[3];
+  for (int k = 0; k < 3; ++k) {
+    oi[k] = ri[k] - vi[k] * dt;
+    oj[k] = rj[k] - vj[k] * dt;
+  }
+  const double vi_ri = dot3(vi, oi);
+  const double vi_rj = dot3(vi, oj);
+  const double vj_ri = dot3(vj, oi);
+  const double vj_rj = dot3(vj, oj);
+  const double nominator = vi_rj + vj_ri - vi_ri - vj_rj;
+  const double two_vi_vj = 2.0 * dot3(vi, vj);
+  const double vi_square = dot3(vi, vi);
+  const double vj_square = dot3(vj, vj);
+  const double denominator = vi_square + vj_square - two_vi_vj;
+  double t = nominator / denominator;
+  /* std::clamp(t, 0., dt): NaN passes through */
+  t = (t < 0.0) ? 0.0 : ((dt < t) ? dt : t);
+  for (int k = 0; k < 3; ++k) {
+    const double p1k = oi[k] + vi[k] * t;
+    const double p2k = oj[k] + vj[k] * t;
+    dr[k] = p1k - p2k;
+    p2[k] = p2k;
+  }
+}
+
+int or_collision_functor(const or_objects *o, size_t i, size_t j, const or_functor_cfg *cfg, or_pair *out) {
+  /* CollisionFunctor.cpp:40-131 (constructor scalings :24-28) */
+  const double cutoff_sq = cfg->cutoff_km * cfg->cutoff_km;
+  const double cdf = cfg->collision_distance_factor / 1000.;
+  const double evasion_sq = cfg->evasion_cutoff_km * cfg->evasion_cutoff_km;
+  const int ai = o->activity[i], aj = o->activity[j];
+  const double Ri = o->radius_m[i], Rj = o->radius_m[j];
+  int evaded = 0;
+  if ((ai != OR_PASSIVE && aj != OR_PASSIVE) || (ai != OR_PASSIVE && Rj >= cfg->min_detection_radius_m) ||
+      (aj != OR_PASSIVE && Ri >= cfg->min_detection_radius_m))
+    evaded = 1;
+  if (o->parent[i] != UINT64_MAX && o->parent[i] == o->parent[j]) return 0;
+  const double ri[3] = {o->rx[i], o->ry[i], o->rz[i]};
+  const double rj[3] = {o->rx[j], o->ry[j], o->rz[j]};
+  const double d0[3] = {ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2]};
+  const double distanceSquare = dot3(d0, d0);
+  if (distanceSquare > cutoff_sq) return 0;
+  const double vi[3] = {o->vx[i], o->vy[i], o->vz[i]};
+  const double vj[3] = {o->vx[j], o->vy[j], o->vz[j]};
+  double dr[3], p2[3];
+  or_distance_by_linear_interpolation(ri, vi, rj, vj, cfg->dt, dr, p2);
+  const double dl = dot3(dr, dr);
+  const double scaled = (Ri + Rj) * cdf;
+  if (evaded) {
+    if (dl > evasion_sq) return 0;
+  } else if (cfg->threshold_mode == 1) {
+    if (dl > cfg->threshold_km * cfg->threshold_km) return 0;
+  } else {
+    if (dl > scaled * scaled) return 0;
+  }
+  for (int k = 0; k < 3; ++k) out->mid[k] = p2[k] + dr[k] * 0.5;
+  out->d2 = dl;
+  if (o->id[i] < o->id[j]) {
+    out->id_a = o->id[i];
+    out->id_b = o->id[j];
+    out->idx_a = (uint32_t)i;
+    out->idx_b = (uint32_t)j;
+  } else {
+    out->id_a = o->id[j];
+    out->id_b = o->id[i];
+    out->idx_a = (uint32_t)j;
+    out->idx_b = (uint32_t)i;
+  }
+  return evaded ? 2 : 1;
+}
+
+typedef struct {
+  or_pair *v;
+  size_t n, cap;
+} pvec;
+
+static void pvec_push(pvec *p, const or_pair *x) {
+  if (p->n == p->cap) {
+    p->cap = p->cap ? 2 * p->cap : 64;
+    p->v = (or_pair *)realloc(p->v, p->cap * sizeof(or_pair));
+  }
+  p->v[p->n++] = *x;
+}
+
+static void emit(const or_pair *pr, int kind, or_pair *hits, size_t cap, size_t *n, or_pair *evaded, size_t ecap,
+                 size_t *ne) {
+  if (kind == 1) {
+    if (*n < cap) hits[*n] = *pr;
+    ++*n;
+  } else if (kind == 2) {
+    if (*ne < ecap) evaded[*ne] = *pr;
+    ++*ne;
+  }
+}
+
+void or_pairs_bruteforce(const or_objects *o, const or_functor_cfg *cfg, or_pair *hits, size_t cap, size_t *n,
+                         or_pair *evaded, size_t ecap, size_t *ne) {
+  *n = 0;
+  *ne = 0;
+  or_pair pr;
+  for (size_t i = 0; i < o->n; ++i)
+    for (size_t j = i + 1; j < o->n; ++j) {
+      const int kind = or_collision_functor(o, i, j, cfg, &pr);
+      emit(&pr, kind, hits, cap, n, evaded, ecap, ne);
+    }
+}
+
+typedef struct {
+  uint64_t key;
+  uint32_t idx;
+} keyidx;
+
+static int cmp_keyidx(const void *a, const void *b) {
+  const keyidx *x = (const keyidx *)a, *y = (const keyidx *)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+#define GRID_BIAS 1048576LL /* 2^20: cell coordinates stored in 21 bits each */
+static uint64_t pack_cell(long long ix, long long iy, long long iz) {
+  return ((uint64_t)(ix + GRID_BIAS) << 42) | ((uint64_t)(iy + GRID_BIAS) << 21) | (uint64_t)(iz + GRID_BIAS);
+}
+
+void or_pairs_grid(const or_objects *o, const or_functor_cfg *cfg, or_pair *hits, size_t cap, size_t *n,
+                   or_pair *evaded, size_t ecap, size_t *ne, int nthreads) {
+  /* linked cells of side cutoff*(1+1e-7): every pair within the cutoff lies in
+   * neighbouring cells; the pair is evaluated from its lower index only. */
+  *n = 0;
+  *ne = 0;
+  const size_t N = o->n;
+  if (N < 2) return;
+  const double s = cfg->cutoff_km * (1.0 + 1e-7);
+  long long *cell = (long long *)malloc(sizeof(long long) * 3 * N);
+  int fallback = !(s > 0.0);
+  for (size_t i = 0; i < N && !fallback; ++i) {
+    const double u[3] = {o->rx[i] / s, o->ry[i] / s, o->rz[i] / s};
+    for (int k = 0; k < 3; ++k) {
+      if (!(fabs(u[k]) < (double)(GRID_BIAS - 2))) {
+        fallback = 1;
+        break;
+      }
+      cell[3 * i + k] = (long long)floor(u[k]);
+    }
+  }
+  if (fallback) {
+    free(cell);
+    or_pairs_bruteforce(o, cfg, hits, cap, n, evaded, ecap, ne);
+    return;
+  }
+  keyidx *ki = (keyidx *)malloc(sizeof(keyidx) * N);
+  for (size_t i = 0; i < N; ++i) {
+    ki[i].key = pack_cell(cell[3 * i], cell[3 * i + 1], cell[3 * i + 2]);
+    ki[i].idx = (uint32_t)i;
+  }
+  qsort(ki, N, sizeof(keyidx), cmp_keyidx);
+  /* unique cells and their ranges */
+  size_t nu = 0;
+  uint64_t *ukey = (uint64_t *)malloc(sizeof(uint64_t) * N);
+  size_t *ubeg = (size_t *)malloc(sizeof(size_t) * (N + 1));
+  for (size_t i = 0; i < N; ++i) {
+    if (i == 0 || ki[i].key != ki[i - 1].key) {
+      ukey[nu] = ki[i].key;
+      ubeg[nu] = i;
+      ++nu;
+    }
+  }
+  ubeg[nu] = N;
+  int nt = 1;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+  pvec *th = (pvec *)calloc((size_t)nt * 2, sizeof(pvec));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256)
+#endif
+  for (long long ii = 0; ii < (long long)N; ++ii) {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    const size_t i = (size_t)ii;
+    or_pair pr;
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          const uint64_t key = pack_cell(cell[3 * i] + dx, cell[3 * i + 1] + dy, cell[3 * i + 2] + dz);
+          size_t lo = 0, hi = nu;
+          while (lo < hi) {
+            const size_t mid = (lo + hi) / 2;
+            if (ukey[mid] < key)
+              lo = mid + 1;
+            else
+              hi = mid;
+          }
+          if (lo == nu || ukey[lo] != key) continue;
+          for (size_t q = ubeg[lo]; q < ubeg[lo + 1]; ++q) {
+            const size_t j = ki[q].idx;
+            if (j <= i) continue;
+            const int kind = or_collision_functor(o, i, j, cfg, &pr);
+            if (kind == 1) pvec_push(&th[2 * tid], &pr);
+            if (kind == 2) pvec_push(&th[2 * tid + 1], &pr);
+          }
+        }
+  }
+  for (int t = 0; t < nt; ++t) {
+    for (size_t k = 0; k < th[2 * t].n; ++k) emit(&th[2 * t].v[k], 1, hits, cap, n, evaded, ecap, ne);
+    for (size_t k = 0; k < th[2 * t + 1].n; ++k) emit(&th[2 * t + 1].v[k], 2, hits, cap, n, evaded, ecap, ne);
+    free(th[2 * t].v);
+    free(th[2 * t + 1].v);
+  }
+  free(th);
+  free(ubeg);
+  free(ukey);
+  free(ki);
+  free(cell);
+}
+
+static int cmp_pair(const void *a, const void *b) {
+  const or_pair *x = (const or_pair *)a, *y = (const or_pair *)b;
+  if (x->id_a != y->id_a) return x->id_a < y->id_a ? -1 : 1;
+  if (x->id_b != y->id_b) return x->id_b < y->id_b ? -1 : 1;
+  if (x->idx_a != y->idx_a) return x->idx_a < y->idx_a ? -1 : 1;
+  if (x->idx_b != y->idx_b) return x->idx_b < y->idx_b ? -1 : 1;
+  return 0;
+}
+
+void or_sort_pairs(or_pair *p, size_t n) { qsort(p, n, sizeof(or_pair), cmp_pair); }
+
+static uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+void or_pairset_digest(const or_pair *p, size_t n, uint64_t digest[2]) {
+  /* multiset hash: two independent 64-bit sums of a mixed pair hash */
+  uint64_t s0 = 0, s1 = 0;
+  for (size_t k = 0; k < n; ++k) {
+    const uint64_t h = splitmix64(p[k].id_a ^ splitmix64(p[k].id_b));
+    s0 += h;
+    s1 += splitmix64(h ^ 0xD1B54A32D192ED03ull);
+  }
+  digest[0] = s0;
+  digest[1] = s1;
+}
