@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Headline benchmark: object-steps/s of the propagate + conjunction hot path.
+
+Workload (one GPU): 1,048,576 objects of the debris-tail population, full force
+model (Kepler, J2, C22, S22, Sun, Moon, SRP, drag), dt = 10 s, linked-cell
+conjunction scan every step with a 10 km cutoff and a 1 km absolute threshold.
+A step is one iteration of the simulation loop: Yoshida step of every object,
+burn-up below 150 km (Simulation::deleteBurnUps), cell binning, pair scan with
+the exact FP64 closest-approach test.
+
+Both precision forms run the same steps from the same initial state:
+  value          object-steps/s of the split form (FP32 perturbations and prefilter)
+  fp64_value     the plain FP64 form
+  pos_err_cm_*   split vs FP64 position difference after the timed steps
+  pairs_equal    whether both forms report the same (iteration, id pair) set
+
+Inputs are resident in HBM before the timed region; the timed region is
+bracketed by stream synchronisations.  The roofline entry covers the kernel
+with the largest device time in a profiled pass (HIP events around each launch
+on the engine's stream).  The CPU baseline times the C oracle (restatement of
+the reference path, OpenMP) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "object-steps/s @1M objects: split-prec vs FP64 GPU at ≤1 cm/day pos error"
+MIN_ALT = 150.0  # sim/minAltitude default (cfg/default_cfg.yaml)
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# Algorithmic HBM bytes per object and launch (per object-step for propagate),
+# counted from the kernels' own loads and stores.
+BYTES = {
+    # r, v read + write (96) + srp/drag coefficients (16 FP64 | 8 FP32) + activity (4) + slot (4) + count atomic (4)
+    "propagate": {"fp64": 96 + 16 + 4 + 4 + 4, "split": 96 + 8 + 4 + 4 + 4},
+    # r read (24) + slot read (4) + bucket start (4) + record write (32 | 16) + index write (4)
+    "bin_scatter": {"fp64": 24 + 4 + 4 + 32 + 4, "split": 24 + 4 + 4 + 16 + 4},
+    # own record (32 | 16) + own index (4) + 4 run bounds (32); neighbour records are hits in L2
+    "pair_scan": {"fp64": 32 + 4 + 32, "split": 16 + 4 + 32},
+}
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=8640, help="timed steps (8640 = one day of 10 s steps)")
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--objects", type=int, default=1 << 20)
+    p.add_argument("--profile-steps", type=int, default=500)
+    p.add_argument("--cpu-steps", type=int, default=12)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--forms", default="fp64,split")
+    return p.parse_args()
+
+
+def make_engine(la, precision):
+    cfg = la.EngineConfig(dt=10.0, components=la.ALL_COMPONENTS, precision=precision, cutoff_km=10.0,
+                          threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, log_capacity=1 << 24)
+    return la.Engine(cfg)
+
+
+def run_form(la, pop, precision, steps, warmup, profile_steps):
+    eng = make_engine(la, precision)
+    eng.set_objects(pop)
+    if warmup:
+        eng.simulate(warmup, first_iteration=0, tpcd=1, min_altitude_km=MIN_ALT)
+    eng.clear_log()
+    eng.set_objects(pop)
+    eng.set_time(0.0)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    stats = eng.simulate(steps, first_iteration=0, tpcd=1, min_altitude_km=MIN_ALT)
+    eng.synchronize()
+    elapsed = time.perf_counter() - t0
+    r, _ = eng.state()
+    ids = eng.ids()
+    log = eng.fetch_log()
+    # profiled pass: per-kernel device time from HIP events on the engine's stream
+    eng.clear_log()
+    eng.set_objects(pop)
+    eng.set_time(0.0)
+    eng.set_profiling(True)
+    eng.simulate(profile_steps, first_iteration=0, tpcd=1, min_altitude_km=MIN_ALT)
+    kernels = {}
+    for name in ("propagate", "bin_reduce", "bin_scan", "bin_scatter", "pair_scan"):
+        launches, ms = eng.kernel_stats(name)
+        if launches:
+            kernels[name] = {"launches": launches, "avg_us": 1000.0 * ms / launches}
+    eng.set_profiling(False)
+    eng.close()
+    return {"elapsed": elapsed, "stats": stats, "r": r, "ids": ids, "log": log, "kernels": kernels}
+
+
+def cpu_baseline(pop, steps):
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as ol
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    n = pop.n
+    r, v = pop.r(), pop.v()
+    cfg = ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        r, v = ol.propagate(r, v, s * 10.0, 10.0, 0xFF, 1, pop.aom, pop.bc_inv, pop.cr, pop.activity, threads)
+        ol.pairs(ol.ObjectSet(r, v, pop.radius_m, pop.activity, pop.id, pop.parent), cfg, "grid", threads)
+    el = time.perf_counter() - t0
+    return {"value": n * steps / el, "unit": "object-steps/s", "cores": threads, "kind": "port",
+            "sample": f"C oracle (OpenMP, -O2, no FMA contraction): {n} objects x {steps} steps of the same "
+                      f"workload (propagate + linked-cell scan), {el:.1f} s on {threads} host threads"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the engine has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+    import ladds_amd as la
+
+    if la.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    t_gen = time.perf_counter()
+    pop = la.generate_population(1, 0x1ADD5 + 1, a.objects)
+    print(f"# population {a.objects} generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    forms = {"fp64": la.FP64, "split": la.SPLIT}
+    res = {}
+    for name in a.forms.split(","):
+        res[name] = run_form(la, pop, forms[name], a.steps, a.warmup, a.profile_steps)
+        r = res[name]
+        print(f"# {name}: {a.steps} steps in {r['elapsed']:.3f} s, {a.objects * a.steps / r['elapsed']:.4g} "
+              f"object-steps/s, collisions {r['stats']['collisions']}, candidates {r['stats']['candidates']}, "
+              f"kernels {json.dumps(r['kernels'])}", file=sys.stderr, flush=True)
+
+    main_form = "split" if "split" in res else next(iter(res))
+    m = res[main_form]
+    value = a.objects * a.steps / m["elapsed"]
+    out = {
+        "metric": METRIC, "value": value, "unit": "object-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1000.0 * m["elapsed"] / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32" if main_form == "split" else "f64",
+        "data": "synthetic (seeded debris-tail population)",
+        "config": {"workload": f"{a.objects} objects, full force model, dt 10 s, {a.steps} steps, cutoff 10 km, "
+                               f"1 km absolute threshold, detection every step",
+                   "objects": a.objects, "precision": main_form},
+    }
+    if "fp64" in res and "split" in res:
+        f = res["fp64"]
+        out["fp64_value"] = a.objects * a.steps / f["elapsed"]
+        out["split_over_fp64"] = out["value"] / out["fp64_value"]
+        sp = res["split"]
+        common, i_s, i_f = np.intersect1d(sp["ids"], f["ids"], return_indices=True)
+        err = np.linalg.norm(sp["r"][i_s] - f["r"][i_f], axis=1) * 1e5
+        out["pos_err_cm_max"] = float(err.max())
+        out["pos_err_cm_median"] = float(np.median(err))
+        out["remaining"] = {"split": int(sp["ids"].shape[0]), "fp64": int(f["ids"].shape[0]),
+                            "common": int(common.shape[0])}
+
+        def pairkeys(log):
+            p = log["pair"]
+            keys = np.stack([log["iteration"], p["id_a"], p["id_b"], log["evaded"].astype(np.uint64)], 1)
+            return keys[np.lexsort(keys.T[::-1])]
+
+        ks, kf = pairkeys(res["split"]["log"]), pairkeys(f["log"])
+        out["pairs_equal"] = bool(ks.shape == kf.shape and np.array_equal(ks, kf))
+        out["pairs"] = {"split": int(ks.shape[0]), "fp64": int(kf.shape[0])}
+    # roofline of the dominant kernel of the headline form
+    kern = m["kernels"]
+    dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
+    per_obj = BYTES.get(dom, {}).get(main_form)
+    if per_obj:
+        achieved = per_obj * a.objects / (kern[dom]["avg_us"] * 1e-6) / 1e9
+        out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "avg_us": kern[dom]["avg_us"], "bytes_per_object": per_obj}
+    out["kernels_us"] = {form: {k: round(v["avg_us"], 2) for k, v in res[form]["kernels"].items()} for form in res}
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(pop, a.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

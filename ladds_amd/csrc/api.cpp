@@ -65,16 +65,18 @@ struct ladds_ctx {
   DeviceObjects d_alt{};  // compaction target
 
   GridParams grid{};
-  uint32_t *count = nullptr, *slot = nullptr, *start = nullptr, *tile_sum = nullptr, *sidx = nullptr;
+  uint32_t *count = nullptr, *slot = nullptr, *start = nullptr, *tile_sum = nullptr;
   void *rec = nullptr;
   uint32_t *n_alive = nullptr;
   uint32_t grid_buckets_alloc = 0;
 
   SubstepEnv<double> *env64 = nullptr;
   SubstepEnv<float> *env32 = nullptr;
-  size_t env_cap = 0;  // sub-steps
-  std::vector<SubstepEnv<double>> h_env64;
-  std::vector<SubstepEnv<float>> h_env32;
+  // Environments of one segment of up to kEnvSegment steps: computed on the host
+  // into pinned memory and copied asynchronously, so the host prepares chunk k+1
+  // while the device runs chunk k.
+  SubstepEnv<double> *h_env64 = nullptr;
+  SubstepEnv<float> *h_env32 = nullptr;
 
   LoggedPair *log = nullptr;
   uint64_t log_cap = 0;
@@ -192,9 +194,10 @@ hipError_t alloc_objects(DeviceObjects &o, size_t cap) {
 }
 
 int setup_grid(ladds_ctx *ctx, size_t n) {
-  const double min_cell = 2.0 * ctx->cfg.cutoff_km * (1.0 + 1e-9);
+  const double min_cell = 2.0 * search_radius(ctx->form, ctx->cfg.cutoff_km) * (1.0 + 1e-9);
   double cell = ctx->cfg.cell_size_km > 0.0 ? ctx->cfg.cell_size_km : min_cell;
-  if (!(cell >= min_cell)) return ctx->fail(LADDS_E_INVALID, "cell_size_km must be >= 2 * cutoff_km");
+  if (!(cell >= min_cell))
+    return ctx->fail(LADDS_E_INVALID, "cell_size_km must be >= 2 * search radius (cutoff, or the split prefilter radius)");
   const uint32_t bits = std::min<uint32_t>(28u, std::max<uint32_t>(12u, ceil_log2(std::max<size_t>(n, 1))));
   GridParams g{};
   g.inv_cell = 1.0 / cell;
@@ -226,32 +229,37 @@ FunctorParams functor_params(const ladds_ctx *ctx, double dt_interp) {
   fp.threshold2 = c.threshold_km * c.threshold_km;
   fp.dt = dt_interp;
   fp.threshold_mode = c.threshold_mode;
-  // FP32 prefilter radius: positions rounded to FP32 are off by <= 2^-24 |x| per axis;
-  // for |x| < 1e5 km the distance error is < 0.014 km, covered by +0.1 km.
-  const double inflated = c.cutoff_km * (1.0 + 1e-5) + 0.1;
+  const double inflated = search_radius(kFormSplit, c.cutoff_km);
   fp.cutoff2_f32 = std::nextafter((float)(inflated * inflated), INFINITY);
   return fp;
 }
 
-int ensure_env(ladds_ctx *ctx, size_t nsteps) {
-  const size_t need = 3 * nsteps;
-  if (need > ctx->env_cap) {
-    hipFree(ctx->env64);
-    hipFree(ctx->env32);
-    HIPCHK(ctx, dalloc(&ctx->env64, need));
-    HIPCHK(ctx, dalloc(&ctx->env32, need));
-    ctx->env_cap = need;
-  }
-  ctx->h_env64.resize(need);
-  ctx->h_env32.resize(need);
+constexpr int kEnvSegment = 8192;  // steps per environment segment
+constexpr int kChunk = 1024;       // steps per host-side preparation chunk
+
+int alloc_env(ladds_ctx *ctx) {
+  if (ctx->env64) return LADDS_OK;
+  const size_t need = 3 * (size_t)kEnvSegment;
+  HIPCHK(ctx, dalloc(&ctx->env64, need));
+  HIPCHK(ctx, dalloc(&ctx->env32, need));
+  HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&ctx->h_env64), sizeof(SubstepEnv<double>) * need, 0));
+  HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&ctx->h_env32), sizeof(SubstepEnv<float>) * need, 0));
+  return LADDS_OK;
+}
+
+// Environments of steps [first, first + count) of the segment starting at the context's
+// current step; enqueues their upload.  Callers synchronise before a new segment reuses
+// the pinned buffer.
+int stage_env(ladds_ctx *ctx, int first, int count) {
   const double tau[3] = {ctx->c[0], ctx->c[0] + ctx->c[1], ctx->c[0] + ctx->c[1] + ctx->c[2]};
-  for (size_t s = 0; s < nsteps; ++s) {
-    const double t = ctx->cfg.t0 + (double)(ctx->step_count + s) * ctx->cfg.dt;
+  for (int s = first; s < first + count; ++s) {
+    const double t = ctx->cfg.t0 + (double)(ctx->step_count + (uint64_t)s) * ctx->cfg.dt;
     for (int k = 0; k < 3; ++k) substep_env(t + tau[k] * ctx->cfg.dt, ctx->h_env64[3 * s + k], ctx->h_env32[3 * s + k]);
   }
-  HIPCHK(ctx, hipMemcpyAsync(ctx->env64, ctx->h_env64.data(), sizeof(SubstepEnv<double>) * need,
+  const size_t off = 3 * (size_t)first, cnt = 3 * (size_t)count;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->env64 + off, ctx->h_env64 + off, sizeof(SubstepEnv<double>) * cnt,
                              hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->env32, ctx->h_env32.data(), sizeof(SubstepEnv<float>) * need,
+  HIPCHK(ctx, hipMemcpyAsync(ctx->env32 + off, ctx->h_env32 + off, sizeof(SubstepEnv<float>) * cnt,
                              hipMemcpyHostToDevice, ctx->stream));
   return LADDS_OK;
 }
@@ -288,7 +296,8 @@ int launch_detection(ladds_ctx *ctx, bool counted, double dt_interp, bool digest
   const FunctorParams fp = functor_params(ctx, dt_interp);
   hipError_t e = hipSuccess;
   if (!counted)
-    e = timed(ctx, "bin_count", [&] { return launch_bin_count(ctx->d, ctx->grid, ctx->count, ctx->slot, ctx->stream); });
+    e = timed(ctx, "bin_count",
+              [&] { return launch_bin_count(ctx->form, ctx->d, ctx->grid, ctx->count, ctx->slot, ctx->stream); });
   if (e == hipSuccess)
     e = timed(ctx, "bin_reduce", [&] { return launch_bin_reduce(ctx->grid, ctx->count, ctx->tile_sum, ctx->stream); });
   if (e == hipSuccess)
@@ -296,12 +305,11 @@ int launch_detection(ladds_ctx *ctx, bool counted, double dt_interp, bool digest
               [&] { return launch_bin_scan(ctx->grid, ctx->count, ctx->tile_sum, ctx->start, ctx->stream); });
   if (e == hipSuccess)
     e = timed(ctx, "bin_scatter", [&] {
-      return launch_bin_scatter(ctx->form, ctx->d, ctx->grid, ctx->slot, ctx->start, ctx->rec, ctx->sidx,
-                                ctx->stream);
+      return launch_bin_scatter(ctx->form, ctx->d, ctx->grid, ctx->slot, ctx->start, ctx->rec, ctx->stream);
     });
   if (e == hipSuccess)
     e = timed(ctx, "pair_scan", [&] {
-      return launch_pair_scan(ctx->form, digest, ctx->d, ctx->grid, fp, ctx->rec, ctx->sidx, ctx->start, ctx->log,
+      return launch_pair_scan(ctx->form, digest, ctx->d, ctx->grid, fp, ctx->rec, ctx->start, ctx->log,
                               ctx->log_cap, ctx->log_count, ctx->counters, iteration, ctx->stream);
     });
   if (e != hipSuccess) return ctx->hipfail(e, "detection launch");
@@ -414,11 +422,12 @@ int ladds_destroy(ladds_ctx *ctx) {
   hipFree(ctx->slot);
   hipFree(ctx->start);
   hipFree(ctx->tile_sum);
-  hipFree(ctx->sidx);
   hipFree(ctx->rec);
   hipFree(ctx->n_alive);
   hipFree(ctx->env64);
   hipFree(ctx->env32);
+  if (ctx->h_env64) hipHostFree(ctx->h_env64);
+  if (ctx->h_env32) hipHostFree(ctx->h_env32);
   hipFree(ctx->log);
   hipFree(ctx->log_count);
   hipFree(ctx->counters);
@@ -447,11 +456,9 @@ int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
     free_objects(ctx->d);
     free_objects(ctx->d_alt);
     hipFree(ctx->slot);
-    hipFree(ctx->sidx);
-    hipFree(ctx->rec);
+      hipFree(ctx->rec);
     HIPCHK(ctx, alloc_objects(ctx->d, n));
     HIPCHK(ctx, dalloc(&ctx->slot, n));
-    HIPCHK(ctx, dalloc(&ctx->sidx, n));
     HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&ctx->rec), n));
     ctx->cap = n;
   }
@@ -542,17 +549,22 @@ int ladds_propagate(ladds_ctx *ctx, int nsteps) {
     return LADDS_OK;
   }
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  constexpr int kChunk = 1024;
-  for (int done = 0; done < nsteps;) {
-    const int k = std::min(kChunk, nsteps - done);
-    int rc = ensure_env(ctx, (size_t)k);
-    if (rc == LADDS_OK) rc = launch_propagate_steps(ctx, 0, k, false, 0.0);
+  {
+    const int rc = alloc_env(ctx);
     if (rc != LADDS_OK) return rc;
-    // the host env buffer is reused by the next chunk: wait for the upload
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->step_count += (uint64_t)k;
-    done += k;
   }
+  for (int seg0 = 0; seg0 < nsteps; seg0 += kEnvSegment) {
+    const int seg = std::min(kEnvSegment, nsteps - seg0);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // pinned environment buffer is free again
+    for (int s = 0; s < seg; s += kChunk) {
+      const int k = std::min(kChunk, seg - s);
+      int rc = stage_env(ctx, s, k);
+      if (rc == LADDS_OK) rc = launch_propagate_steps(ctx, (size_t)s, k, false, 0.0);
+      if (rc != LADDS_OK) return rc;
+    }
+    ctx->step_count += (uint64_t)seg;
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return LADDS_OK;
 }
 
@@ -645,31 +657,36 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   const bool digest = ctx->cfg.pair_output == LADDS_PAIRS_DIGEST;
   const double burnup_r = min_altitude_km > 0.0 ? min_altitude_km + kREarth : 0.0;
   const double burnup_r2 = burnup_r * burnup_r;  // Simulation.cpp:576,583
-  constexpr int kChunk = 1024;
-  int done = 0;
-  while (done < nsteps && ctx->n > 0) {
-    const int chunk = std::min(kChunk, nsteps - done);
-    int rc = ensure_env(ctx, (size_t)chunk);
+  {
+    const int rc = alloc_env(ctx);
     if (rc != LADDS_OK) return rc;
+  }
+  for (int seg0 = 0; seg0 < nsteps && ctx->n > 0; seg0 += kEnvSegment) {
+    const int seg = std::min(kEnvSegment, nsteps - seg0);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // pinned environment buffer is free again
     int s = 0;
-    while (s < chunk) {
+    int staged = 0;
+    while (s < seg) {
+      if (s >= staged) {
+        const int k = std::min(kChunk, seg - staged);
+        const int rc = stage_env(ctx, staged, k);
+        if (rc != LADDS_OK) return rc;
+        staged += k;
+      }
       // integrate iterations up to (and including) the next detection iteration
       int k = 1;
-      uint64_t it = first_iteration + (uint64_t)(done + s);
-      while (s + k < chunk && it % (uint64_t)tpcd != 0) {
+      uint64_t it = first_iteration + (uint64_t)(seg0 + s);
+      while (s + k < staged && it % (uint64_t)tpcd != 0) {
         ++k;
         ++it;
       }
       const bool detect = (it % (uint64_t)tpcd) == 0;
-      rc = launch_propagate_steps(ctx, (size_t)s, k, detect, burnup_r2);
-      if (rc == LADDS_OK && detect)
-        rc = launch_detection(ctx, true, ctx->cfg.dt * (double)tpcd, digest, it);
+      int rc = launch_propagate_steps(ctx, (size_t)s, k, detect, burnup_r2);
+      if (rc == LADDS_OK && detect) rc = launch_detection(ctx, true, ctx->cfg.dt * (double)tpcd, digest, it);
       if (rc != LADDS_OK) return rc;
       s += k;
     }
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->step_count += (uint64_t)chunk;
-    done += chunk;
+    ctx->step_count += (uint64_t)seg;
   }
   ScanCounters sc{};
   unsigned long long cnt = 0;

@@ -1,11 +1,13 @@
 // binning.h -- cell assignment shared by the propagation (count pass) and the scatter.
 //
-// Cells have side s >= 2 * cutoff.  An object at fractional cell position f < 1/2
-// (per axis) can only have partners within the cutoff in its own cell or the one
-// below, otherwise in its own cell or the one above; its search block is the
-// 2x2x2 cells starting at (ix - [f < 1/2], ...).  Any pair within the cutoff lies
-// in each other's search block, so scanning the 8 buckets of the block from the
-// lower sorted position finds every pair exactly once.
+// Cells have side s >= 2 * R, R the search radius (the cutoff in the FP64 form, the
+// inflated FP32 prefilter radius in the split form).  An object at fractional cell
+// position f < 1/2 (per axis) can only have partners within R in its own cell or the
+// one below, otherwise in its own cell or the one above; its search block is the
+// 2x2x2 cells starting at (ix - [f < 1/2], ...).  Any pair within R lies in each
+// other's search block, so scanning the 8 buckets of the block from the lower sorted
+// position finds every pair exactly once.  The split form bins the FP32-rounded
+// position it also stores in its record, so binning and search use the same point.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,6 +30,17 @@ struct CellInfo {
   uint32_t bucket;  // folded bucket of the object's own cell
   uint32_t base;    // folded lower corner of its 2x2x2 search block, packed (fx, fy, fz)
 };
+
+// The point an object is binned by: its FP64 position, or its FP32-rounded position
+// in the split form.
+template <bool F32>
+__device__ __forceinline__ void binning_point(double &x, double &y, double &z) {
+  if constexpr (F32) {
+    x = (double)(float)x;
+    y = (double)(float)y;
+    z = (double)(float)z;
+  }
+}
 
 __device__ __forceinline__ int32_t cell_axis(double x, double inv_cell, uint32_t &low) {
   double u = x * inv_cell;

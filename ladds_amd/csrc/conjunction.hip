@@ -50,11 +50,14 @@ __device__ __forceinline__ uint32_t block_reduce_sum(uint32_t v, uint32_t *lds) 
 // ---------------------------------------------------------------------------
 // binning
 // ---------------------------------------------------------------------------
+template <bool F32>
 __global__ __launch_bounds__(256) void k_bin_count(const double *rx, const double *ry, const double *rz, size_t n,
                                                    GridParams g, uint32_t *count, uint32_t *slot) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const CellInfo c = cell_of(rx[i], ry[i], rz[i], g);
+  double x = rx[i], y = ry[i], z = rz[i];
+  binning_point<F32>(x, y, z);
+  const CellInfo c = cell_of(x, y, z, g);
   slot[i] = atomicAdd(&count[c.bucket], 1u);
 }
 
@@ -110,20 +113,21 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan(uint32_t *count, cons
 template <typename REC>
 __global__ __launch_bounds__(256) void k_bin_scatter(const double *rx, const double *ry, const double *rz, size_t n,
                                                      GridParams g, const uint32_t *slot, const uint32_t *start,
-                                                     REC *rec, uint32_t *sidx) {
+                                                     REC *rec) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double x = rx[i], y = ry[i], z = rz[i];
+  constexpr bool f32 = sizeof(REC) == sizeof(RecordF32);
+  double x = rx[i], y = ry[i], z = rz[i];
+  binning_point<f32>(x, y, z);
   const CellInfo c = cell_of(x, y, z, g);
   const uint32_t p = start[c.bucket] + slot[i];
   REC r;
   r.x = x;
   r.y = y;
   r.z = z;
-  r.base = c.base;
-  if constexpr (sizeof(REC) == sizeof(RecordF64)) r.pad = 0;
+  r.idx = (uint32_t)i;
+  if constexpr (!f32) r.pad = 0;
   rec[p] = r;
-  sidx[p] = (uint32_t)i;
 }
 
 // ---------------------------------------------------------------------------
@@ -211,7 +215,6 @@ struct PairScanArgs {
   GridParams g;
   FunctorParams fp;
   const void *rec;
-  const uint32_t *sidx;
   const uint32_t *start;
   LoggedPair *log;
   uint64_t log_cap;
@@ -277,14 +280,16 @@ __device__ __forceinline__ void handle_candidate(const PairScanArgs &a, uint32_t
 
 template <int FORM>
 __device__ __forceinline__ bool prefilter(const PairScanArgs &a, const void *recs, uint32_t q, double px, double py,
-                                          double pz, float fx, float fy, float fz) {
+                                          double pz, float fx, float fy, float fz, uint32_t &oj) {
   if constexpr (FORM == kFormFp64) {
     const RecordF64 r = static_cast<const RecordF64 *>(recs)[q];
     const double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+    oj = r.idx;
     return !(dot3(dx, dy, dz, dx, dy, dz) > a.fp.cutoff2);
   } else {
     const RecordF32 r = static_cast<const RecordF32 *>(recs)[q];
     const float dx = fx - r.x, dy = fy - r.y, dz = fz - r.z;
+    oj = r.idx;
     return !(dx * dx + dy * dy + dz * dz > a.fp.cutoff2_f32);
   }
 }
@@ -296,21 +301,24 @@ __global__ __launch_bounds__(256) void k_pair_scan(PairScanArgs a) {
   if (p < a.o.n) {
     double px = 0, py = 0, pz = 0;
     float fx = 0, fy = 0, fz = 0;
-    uint32_t base;
+    uint32_t oi;
     if constexpr (FORM == kFormFp64) {
       const RecordF64 r = static_cast<const RecordF64 *>(a.rec)[p];
       px = r.x;
       py = r.y;
       pz = r.z;
-      base = r.base;
+      oi = r.idx;
     } else {
       const RecordF32 r = static_cast<const RecordF32 *>(a.rec)[p];
       fx = r.x;
       fy = r.y;
       fz = r.z;
-      base = r.base;
+      px = fx;
+      py = fy;
+      pz = fz;
+      oi = r.idx;
     }
-    const uint32_t oi = a.sidx[p];
+    const uint32_t base = cell_of(px, py, pz, a.g).base;
     const uint32_t mx = (1u << a.g.bx) - 1u, my = (1u << a.g.by) - 1u, mz = (1u << a.g.bz) - 1u;
     const uint32_t sx = a.g.by + a.g.bz, sy = a.g.bz;
     const uint32_t bxs = base >> sx, bys = (base >> sy) & my, bzs = base & mz;
@@ -328,7 +336,8 @@ __global__ __launch_bounds__(256) void k_pair_scan(PairScanArgs a) {
         uint32_t q = lo > p + 1 ? lo : p + 1;
 #pragma unroll 1
         for (; q < hi; ++q) {
-          if (prefilter<FORM>(a, a.rec, q, px, py, pz, fx, fy, fz)) handle_candidate<DIGEST>(a, oi, a.sidx[q], acc);
+          uint32_t oj;
+          if (prefilter<FORM>(a, a.rec, q, px, py, pz, fx, fy, fz, oj)) handle_candidate<DIGEST>(a, oi, oj, acc);
         }
         if (bz1 != 0u) break;
         lo = a.start[row];
@@ -361,11 +370,14 @@ __global__ __launch_bounds__(256) void k_pair_scan(PairScanArgs a) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_bin_count(const DeviceObjects &o, const GridParams &g, uint32_t *count, uint32_t *slot,
+hipError_t launch_bin_count(int form, const DeviceObjects &o, const GridParams &g, uint32_t *count, uint32_t *slot,
                             hipStream_t s) {
   if (o.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bin_count, dim3((unsigned)((o.n + 255) / 256)), dim3(256), 0, s, o.rx, o.ry, o.rz, o.n, g,
-                     count, slot);
+  const dim3 grid((unsigned)((o.n + 255) / 256));
+  if (form == kFormFp64)
+    hipLaunchKernelGGL(k_bin_count<false>, grid, dim3(256), 0, s, o.rx, o.ry, o.rz, o.n, g, count, slot);
+  else
+    hipLaunchKernelGGL(k_bin_count<true>, grid, dim3(256), 0, s, o.rx, o.ry, o.rz, o.n, g, count, slot);
   return hipGetLastError();
 }
 
@@ -382,20 +394,20 @@ hipError_t launch_bin_scan(const GridParams &g, uint32_t *count, const uint32_t 
 }
 
 hipError_t launch_bin_scatter(int form, const DeviceObjects &o, const GridParams &g, const uint32_t *slot,
-                              const uint32_t *start, void *rec, uint32_t *sidx, hipStream_t s) {
+                              const uint32_t *start, void *rec, hipStream_t s) {
   if (o.n == 0) return hipSuccess;
   const dim3 grid((unsigned)((o.n + 255) / 256));
   if (form == kFormFp64)
     hipLaunchKernelGGL(k_bin_scatter<RecordF64>, grid, dim3(256), 0, s, o.rx, o.ry, o.rz, o.n, g, slot, start,
-                       static_cast<RecordF64 *>(rec), sidx);
+                       static_cast<RecordF64 *>(rec));
   else
     hipLaunchKernelGGL(k_bin_scatter<RecordF32>, grid, dim3(256), 0, s, o.rx, o.ry, o.rz, o.n, g, slot, start,
-                       static_cast<RecordF32 *>(rec), sidx);
+                       static_cast<RecordF32 *>(rec));
   return hipGetLastError();
 }
 
 hipError_t launch_pair_scan(int form, bool digest, const DeviceObjects &o, const GridParams &g,
-                            const FunctorParams &fp, const void *rec, const uint32_t *sidx, const uint32_t *start,
+                            const FunctorParams &fp, const void *rec, const uint32_t *start,
                             LoggedPair *log, uint64_t log_cap, unsigned long long *log_count,
                             ScanCounters *counters, uint64_t iteration, hipStream_t s) {
   if (o.n == 0) return hipSuccess;
@@ -404,7 +416,6 @@ hipError_t launch_pair_scan(int form, bool digest, const DeviceObjects &o, const
   a.g = g;
   a.fp = fp;
   a.rec = rec;
-  a.sidx = sidx;
   a.start = start;
   a.log = log;
   a.log_cap = log_cap;

@@ -25,10 +25,12 @@ struct SubstepEnv {
   T moon[3];              // Moon position [km]
   T moon_inv_s2;          // 1 / |m|^2
   T moon_k;               // GM_moon / |m|^3
-  T pad[4];
+  T sun_acc[3];           // GM_sun s / |s|^3   (indirect term of the direct third-body form)
+  T moon_acc[3];          // GM_moon m / |m|^3
+  T pad[2];
 };
-static_assert(sizeof(SubstepEnv<double>) == 16 * 8, "env layout");
-static_assert(sizeof(SubstepEnv<float>) == 16 * 4, "env layout");
+static_assert(sizeof(SubstepEnv<double>) == 20 * 8, "env layout");
+static_assert(sizeof(SubstepEnv<float>) == 20 * 4, "env layout");
 
 // Folded spatial grid: cell (ix, iy, iz) -> bucket ((ix & mx) * Ky + (iy & my)) * Kz + (iz & mz).
 // Neighbouring cells stay adjacent in bucket order (z fastest), so buckets hold
@@ -40,16 +42,16 @@ struct GridParams {
   uint32_t nbuckets;     // Kx * Ky * Kz
 };
 
-// FP64 sorted record (plain FP64 form) and FP32 record (split form).  `base`
-// packs the folded lower corner (fx, fy, fz) of the object's 2x2x2 search block.
+// Sorted record of the plain FP64 form and of the split form: the position the
+// object is binned by, and its index in the object arrays.
 struct alignas(32) RecordF64 {
   double x, y, z;
-  uint32_t base;
+  uint32_t idx;
   uint32_t pad;
 };
 struct alignas(16) RecordF32 {
   float x, y, z;
-  uint32_t base;
+  uint32_t idx;
 };
 
 // Device view of the object arrays.

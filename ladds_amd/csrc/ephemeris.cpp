@@ -80,6 +80,10 @@ void substep_env(double t, SubstepEnv<double> &e64, SubstepEnv<float> &e32) {
   e64.sun_k = kGmSun / (s2 * std::sqrt(s2));
   e64.moon_inv_s2 = 1.0 / m2;
   e64.moon_k = kGmMoon / (m2 * std::sqrt(m2));
+  for (int k = 0; k < 3; ++k) {
+    e64.sun_acc[k] = e64.sun_k * e64.sun[k];
+    e64.moon_acc[k] = e64.moon_k * e64.moon[k];
+  }
   for (double &x : e64.pad) x = 0.0;
   e32.ct = (float)e64.ct;
   e32.st = (float)e64.st;
@@ -91,6 +95,10 @@ void substep_env(double t, SubstepEnv<double> &e64, SubstepEnv<float> &e32) {
   e32.sun_k = (float)e64.sun_k;
   e32.moon_inv_s2 = (float)e64.moon_inv_s2;
   e32.moon_k = (float)e64.moon_k;
+  for (int k = 0; k < 3; ++k) {
+    e32.sun_acc[k] = (float)e64.sun_acc[k];
+    e32.moon_acc[k] = (float)e64.moon_acc[k];
+  }
   for (float &x : e32.pad) x = 0.f;
 }
 

@@ -14,17 +14,25 @@ hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<d
                             const double ddt[3], const GridParams *grid, uint32_t *count, uint32_t *slot,
                             double burnup_r2, hipStream_t stream);
 
-hipError_t launch_bin_count(const DeviceObjects &o, const GridParams &g, uint32_t *count, uint32_t *slot,
+hipError_t launch_bin_count(int form, const DeviceObjects &o, const GridParams &g, uint32_t *count, uint32_t *slot,
                             hipStream_t s);
 hipError_t launch_bin_reduce(const GridParams &g, const uint32_t *count, uint32_t *tile_sum, hipStream_t s);
 hipError_t launch_bin_scan(const GridParams &g, uint32_t *count, const uint32_t *tile_sum, uint32_t *start,
                            hipStream_t s);
 hipError_t launch_bin_scatter(int form, const DeviceObjects &o, const GridParams &g, const uint32_t *slot,
-                              const uint32_t *start, void *rec, uint32_t *sidx, hipStream_t s);
+                              const uint32_t *start, void *rec, hipStream_t s);
 hipError_t launch_pair_scan(int form, bool digest, const DeviceObjects &o, const GridParams &g,
-                            const FunctorParams &fp, const void *rec, const uint32_t *sidx, const uint32_t *start,
+                            const FunctorParams &fp, const void *rec, const uint32_t *start,
                             LoggedPair *log, uint64_t log_cap, unsigned long long *log_count,
                             ScanCounters *counters, uint64_t iteration, hipStream_t s);
+
+// Search radius of the pair scan: the cutoff in the FP64 form; in the split form the
+// FP32 prefilter radius, inflated so that FP32 rounding of positions (< 2^-24 |x| per
+// axis, < 0.014 km for |x| < 1e5 km) and of the FP32 distance cannot reject a pair
+// the exact FP64 test keeps.
+inline double search_radius(int form, double cutoff_km) {
+  return form == kFormFp64 ? cutoff_km : cutoff_km * (1.0 + 1e-5) + 0.1;
+}
 
 // Burn-up compaction: alive flags are activity >= 0.
 hipError_t launch_count_alive(const DeviceObjects &o, uint32_t *tile_sum, uint32_t ntiles, hipStream_t s);

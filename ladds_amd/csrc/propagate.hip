@@ -21,9 +21,7 @@
 
 namespace ladds {
 
-__constant__ double c_atm_h0[kAtmRows];
-__constant__ double c_atm_rho[kAtmRows];
-__constant__ double c_atm_inv_scale[kAtmRows];
+__constant__ AtmTable c_atm;
 
 struct PropagateArgs {
   DeviceObjects o;
@@ -40,6 +38,13 @@ struct PropagateArgs {
 
 template <int FORM>
 __global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
+  __shared__ AtmTable atm;
+  {
+    const double *src = reinterpret_cast<const double *>(&c_atm);
+    double *dst = reinterpret_cast<double *>(&atm);
+    for (int k = threadIdx.x; k < (int)(sizeof(AtmTable) / sizeof(double)); k += blockDim.x) dst[k] = src[k];
+    __syncthreads();
+  }
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.o.n) return;
   double x = a.o.rx[i], y = a.o.ry[i], z = a.o.rz[i];
@@ -48,14 +53,13 @@ __global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
   p.activity = a.o.activity[i];
   // dummies (burnt-up objects awaiting compaction) keep their state
   const int nsteps = p.activity < 0 ? 0 : a.nsteps;
+  p.drag64 = a.o.drag64[i];
   if constexpr (FORM == kFormFp64) {
     p.srp64 = a.o.srp64[i];
-    p.drag64 = a.o.drag64[i];
-    p.srp32 = p.drag32 = 0.f;
+    p.srp32 = 0.f;
   } else {
     p.srp32 = a.o.srp32[i];
-    p.drag32 = a.o.drag32[i];
-    p.srp64 = p.drag64 = 0.0;
+    p.srp64 = 0.0;
   }
   bool burnt = false;
   for (int s = 0; s < nsteps; ++s) {
@@ -66,7 +70,8 @@ __global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
       y += cdt * vy;
       z += cdt * vz;
       double ax, ay, az;
-      acceleration<FORM>(x, y, z, vx, vy, vz, p, a.mask, a.env64[3 * s + k], a.env32[3 * s + k], ax, ay, az);
+      acceleration<FORM>(x, y, z, vx, vy, vz, p, a.mask, a.env64[3 * s + k], a.env32[3 * s + k], atm, ax, ay,
+                         az);
       vx += ddt * ax;
       vy += ddt * ay;
       vz += ddt * az;
@@ -88,18 +93,20 @@ __global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
   a.o.vy[i] = vy;
   a.o.vz[i] = vz;
   if (a.count) {
+    binning_point<FORM != kFormFp64>(x, y, z);
     const CellInfo c = cell_of(x, y, z, a.grid);
     a.slot[i] = atomicAdd(&a.count[c.bucket], 1u);
   }
 }
 
 hipError_t upload_atmosphere() {
-  double inv[kAtmRows];
-  for (int i = 0; i < kAtmRows; ++i) inv[i] = 1.0 / kAtmScale[i];
-  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_atm_h0), kAtmH0, sizeof(kAtmH0));
-  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_atm_rho), kAtmRho, sizeof(kAtmRho));
-  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_atm_inv_scale), inv, sizeof(inv));
-  return e;
+  AtmTable t;
+  for (int i = 0; i < kAtmPadded; ++i) t.h0[i] = i < kAtmRows ? kAtmH0[i] : INFINITY;
+  for (int i = 0; i < kAtmRows; ++i) {
+    t.rho[i] = kAtmRho[i];
+    t.inv_scale[i] = 1.0 / kAtmScale[i];
+  }
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_atm), &t, sizeof(t));
 }
 
 hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<double> *env64,

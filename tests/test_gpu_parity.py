@@ -121,10 +121,10 @@ def test_split_precision_error_after_one_day(precision, limit_km):
 # ---------------------------------------------------------------------------
 # conjunctions
 # ---------------------------------------------------------------------------
-def _cloud(n, seed, box=200.0, centre=(6000.0, 3000.0, 1500.0)):
+def _cloud(n, seed, box=200.0, centre=(6000.0, 3000.0, 1500.0), vel_sigma=5.0):
     rng = np.random.default_rng(seed)
     r = np.array(centre) + rng.uniform(-box / 2, box / 2, (n, 3))
-    v = rng.normal(0, 5, (n, 3))
+    v = np.array([0.0, 4.0, 5.0]) + rng.normal(0, vel_sigma, (n, 3))
     pop = la.Population(r[:, 0].copy(), r[:, 1].copy(), r[:, 2].copy(), v[:, 0].copy(), v[:, 1].copy(),
                         v[:, 2].copy(), rng.uniform(0.01, 600.0, n), np.ones(n), np.zeros(n), np.zeros(n),
                         np.ones(n))
@@ -145,13 +145,14 @@ def test_conjunctions_bit_identical(precision, mode, thr):
         hits, ev = eng.conjunctions(dt)
     objs = _oracle_objects(pop.r(), pop.v(), pop)
     oh, oe = ol.pairs(objs, ol.functor_cfg(cutoff, dt, 1.5, 300.0, 0.8, mode, thr), method="grid")
-    assert oh.shape[0] > 50 and oe.shape[0] > 50
+    assert oh.shape[0] > 20 and oe.shape[0] > 20
     _pair_equal(ol.sort_pairs(hits), oh)
     _pair_equal(ol.sort_pairs(ev), oe)
 
 
 def test_conjunctions_after_propagation_match_bruteforce():
-    """LEO population with planted pairs: GPU scan of the GPU state == oracle brute force."""
+    """LEO population with planted pairs: GPU scan of the GPU state == oracle brute force.
+    After 3 steps the co-orbital half of the 64 planted pairs is still within 1 km."""
     pop = la.generate_population(0, 0x1ADD5, 4096, planted_pairs=64)
     with engine(dt=10.0, components=la.KEP | la.J2, cutoff_km=10.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
                 threshold_km=1.0) as eng:
@@ -161,7 +162,7 @@ def test_conjunctions_after_propagation_match_bruteforce():
         hits, ev = eng.conjunctions()
     objs = _oracle_objects(r, v, pop)
     oh, oe = ol.pairs(objs, ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0), method="brute")
-    assert oh.shape[0] >= 60
+    assert oh.shape[0] >= 30
     _pair_equal(ol.sort_pairs(hits), oh)
     _pair_equal(ol.sort_pairs(ev), oe)
 
@@ -334,7 +335,7 @@ def test_delete_burnups_compacts_in_order():
 
 
 def test_log_overflow_is_reported():
-    pop = _cloud(5000, 13, box=20.0)
+    pop = _cloud(5000, 13, box=20.0, vel_sigma=1e-3)
     with engine(dt=10.0, cutoff_km=2.0, collision_distance_factor=1e6, log_capacity=1024) as eng:
         eng.set_objects(pop)
         stats = eng.simulate(1, tpcd=1)
