@@ -42,8 +42,12 @@ FP64_PEAK_TFLOPS = 78.6
 # Algorithmic HBM bytes per object and launch (per object-step for propagate),
 # counted from the kernels' own loads and stores.
 BYTES = {
-    # r, v read + write (96) + srp/drag coefficients (16 FP64 | 8 FP32) + activity (4) + slot (4) + count atomic (4)
-    "propagate": {"fp64": 96 + 16 + 4 + 4 + 4, "split": 96 + 8 + 4 + 4 + 4},
+    # r, v read + write (96) + srp + drag coefficients (16 FP64 | 4 FP32 + 8 FP64) + activity (4) + cell key (4)
+    "propagate": {"fp64": 96 + 16 + 4 + 4, "split": 96 + 12 + 4 + 4},
+    # keys read (4) + r read (24) + record write (32 | 16)
+    "tile_partition": {"fp64": 4 + 24 + 32, "split": 4 + 24 + 16},
+    # every record read once for its own plane and once as the next plane of the plane below
+    "pair_tiles": {"fp64": 2 * 32, "split": 2 * 16},
     # r read (24) + slot read (4) + bucket start (4) + record write (32 | 16) + index write (4)
     "bin_scatter": {"fp64": 24 + 4 + 4 + 32 + 4, "split": 24 + 4 + 4 + 16 + 4},
     # own record (32 | 16) + own index (4) + 4 run bounds (32); neighbour records are hits in L2
@@ -93,7 +97,8 @@ def run_form(la, pop, precision, steps, warmup, profile_steps):
     eng.set_profiling(True)
     eng.simulate(profile_steps, first_iteration=0, tpcd=1, min_altitude_km=MIN_ALT)
     kernels = {}
-    for name in ("propagate", "bin_reduce", "bin_scan", "bin_scatter", "pair_scan"):
+    for name in ("propagate", "bin_keys", "tile_partition", "pair_tiles", "bin_reduce", "bin_scan", "bin_scatter",
+                 "pair_scan"):
         launches, ms = eng.kernel_stats(name)
         if launches:
             kernels[name] = {"launches": launches, "avg_us": 1000.0 * ms / launches}

@@ -30,6 +30,7 @@ SOURCES = [
     "propagate.hip",
     "conjunction.hip",
     "compact.hip",
+    "tiles.hip",
 ]
 
 COMMON = [

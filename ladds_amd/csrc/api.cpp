@@ -35,8 +35,12 @@ struct KernelProfile {
   double ms = 0.0;  // resolved time
 };
 
-constexpr const char *kKernelNames[] = {"propagate", "bin_count", "bin_reduce",  "bin_scan",
-                                        "bin_scatter", "pair_scan", "compact"};
+constexpr const char *kKernelNames[] = {"propagate",   "bin_count", "bin_reduce",     "bin_scan",
+                                        "bin_scatter", "pair_scan", "bin_keys",       "tile_partition",
+                                        "pair_tiles",  "compact"};
+
+// Pair-scan pipelines (ladds_config.scan_variant).
+enum : int { kVariantAuto = 0, kVariantAtomicSort = 1, kVariantPlaneTiles = 2 };
 
 template <typename T>
 hipError_t dalloc(T **p, size_t count) {
@@ -65,10 +69,16 @@ struct ladds_ctx {
   DeviceObjects d_alt{};  // compaction target
 
   GridParams grid{};
+  TileParams tiles{};
+  int variant = kVariantPlaneTiles;
   uint32_t *count = nullptr, *slot = nullptr, *start = nullptr, *tile_sum = nullptr;
   void *rec = nullptr;
+  uint32_t *tps = nullptr, *arena_top = nullptr;
+  void *scratch = nullptr;
+  size_t tps_alloc = 0;
   uint32_t *n_alive = nullptr;
   uint32_t grid_buckets_alloc = 0;
+  size_t tile_sum_alloc = 0;
 
   SubstepEnv<double> *env64 = nullptr;
   SubstepEnv<float> *env32 = nullptr;
@@ -193,28 +203,87 @@ hipError_t alloc_objects(DeviceObjects &o, size_t cap) {
   return e;
 }
 
-int setup_grid(ladds_ctx *ctx, size_t n) {
-  const double min_cell = 2.0 * search_radius(ctx->form, ctx->cfg.cutoff_km) * (1.0 + 1e-9);
+// Grid geometry.  Atomic counting sort: cells of side >= 2R, 2x2x2 search blocks, about
+// one object per bucket.  Plane tiles: cells of side >= R, 3x3x3 neighbourhoods; enough
+// x-planes for the x-extent of the population (no aliasing of x-slabs), about one object
+// per bucket within a plane, and an LDS capacity sized from the largest plane pair.
+int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
+  const bool tiles = ctx->variant == kVariantPlaneTiles;
+  const double radius = search_radius(ctx->form, ctx->cfg.cutoff_km);
+  const double min_cell = (tiles ? 1.0 : 2.0) * radius * (1.0 + 1e-9);
   double cell = ctx->cfg.cell_size_km > 0.0 ? ctx->cfg.cell_size_km : min_cell;
   if (!(cell >= min_cell))
-    return ctx->fail(LADDS_E_INVALID, "cell_size_km must be >= 2 * search radius (cutoff, or the split prefilter radius)");
-  const uint32_t bits = std::min<uint32_t>(28u, std::max<uint32_t>(12u, ceil_log2(std::max<size_t>(n, 1))));
+    return ctx->fail(LADDS_E_INVALID, tiles ? "cell_size_km must be >= the search radius (cutoff, or the split "
+                                              "prefilter radius)"
+                                            : "cell_size_km must be >= 2 * the search radius");
   GridParams g{};
   g.inv_cell = 1.0 / cell;
-  g.bx = (bits + 2) / 3;
-  g.by = (bits + 1) / 3;
-  g.bz = bits / 3;
+  uint32_t bits;
+  std::vector<uint32_t> plane_count;
+  if (tiles) {
+    double xmax = 1.0e4;
+    if (rx && n) {
+      xmax = 0.0;
+      for (size_t i = 0; i < n; ++i) xmax = std::max(xmax, std::fabs(rx[i]));
+    }
+    const double nx = std::min(1.0e6, 2.0 * xmax / cell + 3.0);
+    g.bx = std::min<uint32_t>(14u, std::max<uint32_t>(2u, ceil_log2((uint64_t)nx)));
+    const uint64_t used = std::min<uint64_t>((uint64_t)nx, 1ull << g.bx);
+    const uint32_t pb = std::min<uint32_t>(12u, std::max<uint32_t>(4u, ceil_log2(std::max<size_t>(n, 1) / used + 1)));
+    g.by = pb / 2;
+    g.bz = pb - g.by;
+    bits = g.bx + pb;
+    if (rx && n) {
+      plane_count.assign(1u << g.bx, 0u);
+      const uint32_t mx = (1u << g.bx) - 1u;
+      for (size_t i = 0; i < n; ++i) {
+        double u = rx[i] * g.inv_cell;
+        u = std::fmin(std::fmax(u, -1.0e9), 1.0e9);
+        plane_count[(uint32_t)(int32_t)std::floor(u) & mx] += 1;
+      }
+    }
+  } else {
+    bits = std::min<uint32_t>(28u, std::max<uint32_t>(12u, ceil_log2(std::max<size_t>(n, 1))));
+    g.bx = (bits + 2) / 3;
+    g.by = (bits + 1) / 3;
+    g.bz = bits / 3;
+  }
   g.nbuckets = 1u << bits;
   ctx->grid = g;
-  if (g.nbuckets > ctx->grid_buckets_alloc) {
+  if (!tiles && g.nbuckets > ctx->grid_buckets_alloc) {
     hipFree(ctx->count);
     hipFree(ctx->start);
-    hipFree(ctx->tile_sum);
     HIPCHK(ctx, dalloc(&ctx->count, g.nbuckets));
     HIPCHK(ctx, dalloc(&ctx->start, (size_t)g.nbuckets + 1));
-    HIPCHK(ctx, dalloc(&ctx->tile_sum, g.nbuckets / 2048 + 1));
     HIPCHK(ctx, hipMemset(ctx->count, 0, sizeof(uint32_t) * g.nbuckets));
     ctx->grid_buckets_alloc = g.nbuckets;
+  }
+  // burn-up compaction reuses tile_sum: one entry per 2048 objects
+  const size_t ts_need = std::max<size_t>((size_t)g.nbuckets / 2048, n / 2048) + 1;
+  if (ts_need > ctx->tile_sum_alloc) {
+    hipFree(ctx->tile_sum);
+    HIPCHK(ctx, dalloc(&ctx->tile_sum, ts_need));
+    ctx->tile_sum_alloc = ts_need;
+  }
+  TileParams tp{};
+  tp.tile_size = kPartitionTile;
+  tp.ntiles = (uint32_t)std::max<size_t>(1, (n + kPartitionTile - 1) / kPartitionTile);
+  tp.plane_bits = g.by + g.bz;
+  tp.nplanes = 1u << g.bx;
+  tp.lds_header_words = pair_tiles_header_words(tp);
+  const size_t rec = (ctx->form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
+  size_t pair_max = 0;
+  for (size_t p = 0; p < plane_count.size(); ++p)
+    pair_max = std::max<size_t>(pair_max, plane_count[p] + plane_count[(p + 1) % plane_count.size()]);
+  const size_t want = std::max<size_t>(1024, ((pair_max + pair_max / 4) + 255) / 256 * 256);
+  const size_t fit = (kPairTileLdsBytes - 4 * (size_t)tp.lds_header_words) / rec;
+  tp.cap_items = (uint32_t)std::min(want, fit);
+  ctx->tiles = tp;
+  const size_t tps_need = (size_t)tp.ntiles * (tp.nplanes + 1);
+  if (tps_need > ctx->tps_alloc) {
+    hipFree(ctx->tps);
+    HIPCHK(ctx, dalloc(&ctx->tps, tps_need));
+    ctx->tps_alloc = tps_need;
   }
   return LADDS_OK;
 }
@@ -278,24 +347,43 @@ int ensure_log(ladds_ctx *ctx, uint64_t cap) {
   return LADDS_OK;
 }
 
-int launch_propagate_steps(ladds_ctx *ctx, size_t env_offset, int nsteps, bool count, double burnup_r2) {
+int launch_propagate_steps(ladds_ctx *ctx, size_t env_offset, int nsteps, bool bin, double burnup_r2) {
   double cdt[4], ddt[3];
   for (int k = 0; k < 4; ++k) cdt[k] = ctx->c[k] * ctx->cfg.dt;
   for (int k = 0; k < 3; ++k) ddt[k] = ctx->dcoef[k] * ctx->cfg.dt;
   const hipError_t e = timed(ctx, "propagate", [&] {
     return launch_propagate(ctx->form, ctx->d, ctx->env64 + 3 * env_offset, ctx->env32 + 3 * env_offset, nsteps,
-                            ctx->cfg.components, cdt, ddt, &ctx->grid, count ? ctx->count : nullptr, ctx->slot,
-                            burnup_r2, ctx->stream);
+                            ctx->cfg.components, cdt, ddt, &ctx->grid,
+                            (bin && ctx->variant == kVariantPlaneTiles) ? ctx->slot : nullptr,
+                            (bin && ctx->variant == kVariantAtomicSort) ? ctx->count : nullptr, ctx->slot, burnup_r2,
+                            ctx->stream);
   });
   if (e != hipSuccess) return ctx->hipfail(e, "propagate launch");
   return LADDS_OK;
 }
 
-// Binning (after the count pass) + pair scan on the current state.
-int launch_detection(ladds_ctx *ctx, bool counted, double dt_interp, bool digest, uint64_t iteration) {
+// Binning (after the propagation's binning pass when `binned`) + pair scan on the current state.
+int launch_detection(ladds_ctx *ctx, bool binned, double dt_interp, bool digest, uint64_t iteration) {
   const FunctorParams fp = functor_params(ctx, dt_interp);
   hipError_t e = hipSuccess;
-  if (!counted)
+  if (ctx->variant == kVariantPlaneTiles) {
+    if (!binned)
+      e = timed(ctx, "bin_keys", [&] { return launch_bin_keys(ctx->form, ctx->d, ctx->grid, ctx->slot, ctx->stream); });
+    if (e == hipSuccess)
+      e = timed(ctx, "tile_partition", [&] {
+        return launch_tile_partition(ctx->form, ctx->d, ctx->slot, ctx->tiles, ctx->rec, ctx->tps, ctx->arena_top,
+                                     ctx->stream);
+      });
+    if (e == hipSuccess)
+      e = timed(ctx, "pair_tiles", [&] {
+        return launch_pair_tiles(ctx->form, digest, ctx->d, ctx->grid, ctx->tiles, fp, ctx->rec, ctx->tps,
+                                 ctx->scratch, ctx->arena_top, ctx->log, ctx->log_cap, ctx->log_count, ctx->counters,
+                                 iteration, ctx->stream);
+      });
+    if (e != hipSuccess) return ctx->hipfail(e, "detection launch");
+    return LADDS_OK;
+  }
+  if (!binned)
     e = timed(ctx, "bin_count",
               [&] { return launch_bin_count(ctx->form, ctx->d, ctx->grid, ctx->count, ctx->slot, ctx->stream); });
   if (e == hipSuccess)
@@ -382,6 +470,7 @@ int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
   if (!cfg || !out) return LADDS_E_INVALID;
   *out = nullptr;
   if (!(cfg->dt > 0.0) || !(cfg->cutoff_km > 0.0) || cfg->precision < 0 || cfg->precision > 2 ||
+      cfg->scan_variant < 0 || cfg->scan_variant > 2 ||
       cfg->threshold_mode < 0 || cfg->threshold_mode > 1 || cfg->pair_output < 0 || cfg->pair_output > 1)
     return LADDS_E_INVALID;
   int ndev = 0;
@@ -390,10 +479,13 @@ int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
   ctx->cfg = *cfg;
   ctx->form = cfg->precision;
   ctx->device = cfg->device;
+  ctx->variant = (cfg->scan_variant == kVariantAtomicSort) ? kVariantAtomicSort : kVariantPlaneTiles;
   yoshida_coefficients(ctx->c, ctx->dcoef);
   hipError_t e = hipSetDevice(ctx->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = upload_atmosphere();
+  if (e == hipSuccess) e = configure_tile_kernels();
+  if (e == hipSuccess) e = dalloc(&ctx->arena_top, 1);
   if (e == hipSuccess) e = dalloc(&ctx->log_count, 1);
   if (e == hipSuccess) e = dalloc(&ctx->counters, 1);
   if (e == hipSuccess) e = dalloc(&ctx->n_alive, 1);
@@ -424,6 +516,9 @@ int ladds_destroy(ladds_ctx *ctx) {
   hipFree(ctx->tile_sum);
   hipFree(ctx->rec);
   hipFree(ctx->n_alive);
+  hipFree(ctx->tps);
+  hipFree(ctx->scratch);
+  hipFree(ctx->arena_top);
   hipFree(ctx->env64);
   hipFree(ctx->env32);
   if (ctx->h_env64) hipHostFree(ctx->h_env64);
@@ -459,11 +554,13 @@ int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
       hipFree(ctx->rec);
     HIPCHK(ctx, alloc_objects(ctx->d, n));
     HIPCHK(ctx, dalloc(&ctx->slot, n));
+    hipFree(ctx->scratch);
     HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&ctx->rec), n));
+    HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&ctx->scratch), 2 * n));
     ctx->cap = n;
   }
   {
-    const int rc = setup_grid(ctx, n);
+    const int rc = setup_grid(ctx, n, o->rx);
     if (rc != LADDS_OK) return rc;
   }
   ctx->n = n;

@@ -20,21 +20,13 @@
 
 #include "binning.h"
 #include "launch.h"
+#include "pairs.h"
 
 namespace ladds {
 
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;  // 2048 buckets per tile
-
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-template <typename T>
-__device__ __forceinline__ T wave_reduce_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 
 __device__ __forceinline__ uint32_t block_reduce_sum(uint32_t v, uint32_t *lds) {
   v = wave_reduce_sum(v);
@@ -130,153 +122,12 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const double *rx, const dou
   rec[p] = r;
 }
 
-// ---------------------------------------------------------------------------
-// exact FP64 predicate: CollisionFunctor::AoSFunctor + distanceByLinearInterpolation
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double dot3(double a0, double a1, double a2, double b0, double b1, double b2) {
-  return exact_dot3(a0, a1, a2, b0, b1, b2);
-}
-
-// Returns 0 (no report), 1 (collision) or 2 (evaded); i < j are object indices, evaluated
-// as AoSFunctor(i, j).
-__device__ __forceinline__ int collision_functor(const DeviceObjects &o, uint32_t i, uint32_t j,
-                                              const FunctorParams &fp, PairRecord &out) {
-#pragma clang fp contract(off)
-  const int ai = o.activity[i], aj = o.activity[j];
-  if (ai < 0 || aj < 0) return 0;  // deleted (dummy) objects, CollisionFunctor.cpp:50-52
-  const double Ri = o.radius_m[i], Rj = o.radius_m[j];
-  const bool evaded = (ai != 0 && aj != 0) || (ai != 0 && Rj >= fp.min_det_radius) ||
-                      (aj != 0 && Ri >= fp.min_det_radius);
-  const uint64_t pi = o.parent[i], pj = o.parent[j];
-  if (pi != ~0ull && pi == pj) return 0;
-  const double rix = o.rx[i], riy = o.ry[i], riz = o.rz[i];
-  const double rjx = o.rx[j], rjy = o.ry[j], rjz = o.rz[j];
-  const double dx = rix - rjx, dy = riy - rjy, dz = riz - rjz;
-  const double d0 = dot3(dx, dy, dz, dx, dy, dz);
-  if (d0 > fp.cutoff2) return 0;
-  const double vix = o.vx[i], viy = o.vy[i], viz = o.vz[i];
-  const double vjx = o.vx[j], vjy = o.vy[j], vjz = o.vz[j];
-  const double dt = fp.dt;
-  // DistanceApproximation.cpp:25-55
-  const double oix = rix - vix * dt, oiy = riy - viy * dt, oiz = riz - viz * dt;
-  const double ojx = rjx - vjx * dt, ojy = rjy - vjy * dt, ojz = rjz - vjz * dt;
-  const double vi_ri = dot3(vix, viy, viz, oix, oiy, oiz);
-  const double vi_rj = dot3(vix, viy, viz, ojx, ojy, ojz);
-  const double vj_ri = dot3(vjx, vjy, vjz, oix, oiy, oiz);
-  const double vj_rj = dot3(vjx, vjy, vjz, ojx, ojy, ojz);
-  const double nominator = vi_rj + vj_ri - vi_ri - vj_rj;
-  const double two_vi_vj = 2.0 * dot3(vix, viy, viz, vjx, vjy, vjz);
-  const double vi_square = dot3(vix, viy, viz, vix, viy, viz);
-  const double vj_square = dot3(vjx, vjy, vjz, vjx, vjy, vjz);
-  const double denominator = vi_square + vj_square - two_vi_vj;
-  double t = nominator / denominator;
-  t = (t < 0.0) ? 0.0 : ((dt < t) ? dt : t);  // std::clamp: NaN passes through
-  const double p2x = ojx + vjx * t, p2y = ojy + vjy * t, p2z = ojz + vjz * t;
-  const double drx = (oix + vix * t) - p2x;
-  const double dry = (oiy + viy * t) - p2y;
-  const double drz = (oiz + viz * t) - p2z;
-  const double dl = dot3(drx, dry, drz, drx, dry, drz);
-  if (evaded) {
-    if (dl > fp.evasion2) return 0;
-  } else if (fp.threshold_mode == 1) {
-    if (dl > fp.threshold2) return 0;
-  } else {
-    const double scaled = (Ri + Rj) * fp.cdf_km;
-    if (dl > scaled * scaled) return 0;
-  }
-  out.mid[0] = p2x + drx * 0.5;
-  out.mid[1] = p2y + dry * 0.5;
-  out.mid[2] = p2z + drz * 0.5;
-  out.d2 = dl;
-  const uint64_t idi = o.id[i], idj = o.id[j];
-  if (idi < idj) {
-    out.id_a = idi;
-    out.id_b = idj;
-    out.idx_a = i;
-    out.idx_b = j;
-  } else {
-    out.id_a = idj;
-    out.id_b = idi;
-    out.idx_a = j;
-    out.idx_b = i;
-  }
-  return evaded ? 2 : 1;
-}
-
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
-}
-
 struct PairScanArgs {
-  DeviceObjects o;
+  PairSink sink;
   GridParams g;
-  FunctorParams fp;
   const void *rec;
   const uint32_t *start;
-  LoggedPair *log;
-  uint64_t log_cap;
-  unsigned long long *log_count;
-  ScanCounters *counters;
-  uint64_t iteration;
 };
-
-struct ScanAcc {
-  unsigned long long cand = 0, hit = 0, ev = 0, d0 = 0, d1 = 0, e0 = 0, e1 = 0;
-};
-
-template <bool DIGEST>
-__device__ __forceinline__ void handle_candidate(const PairScanArgs &a, uint32_t oi, uint32_t oj, ScanAcc &acc) {
-  const uint32_t i = oi < oj ? oi : oj;
-  const uint32_t j = oi < oj ? oj : oi;
-  PairRecord r;
-  const int kind = collision_functor(a.o, i, j, a.fp, r);
-  acc.cand += 1;
-  if constexpr (DIGEST) {
-    if (kind != 0) {
-      const uint64_t h = mix64(r.id_a ^ mix64(r.id_b));
-      const uint64_t h2 = mix64(h ^ 0xD1B54A32D192ED03ull);
-      if (kind == 1) {
-        acc.hit += 1;
-        acc.d0 += h;
-        acc.d1 += h2;
-      } else {
-        acc.ev += 1;
-        acc.e0 += h;
-        acc.e1 += h2;
-      }
-    }
-  } else {
-    // wavefront compaction: one atomic per wavefront for the lanes holding a result
-    const unsigned long long hm = __ballot(kind == 1);
-    const unsigned long long em = __ballot(kind == 2);
-    const unsigned long long all = hm | em;
-    if (all) {
-      const int leader = __ffsll((long long)all) - 1;
-      unsigned long long base = 0;
-      if ((int)lane_id() == leader) {
-        base = atomicAdd(a.log_count, (unsigned long long)__popcll(all));
-        if (hm) atomicAdd(&a.counters->hits, (unsigned long long)__popcll(hm));
-        if (em) atomicAdd(&a.counters->evaded, (unsigned long long)__popcll(em));
-      }
-      base = __shfl(base, leader, 64);
-      if (kind != 0) {
-        const unsigned long long below = all & ((1ull << lane_id()) - 1ull);
-        const unsigned long long pos = base + (unsigned long long)__popcll(below);
-        if (pos < a.log_cap) {
-          LoggedPair lp;
-          lp.pair = r;
-          lp.iteration = a.iteration;
-          lp.evaded = (kind == 2) ? 1u : 0u;
-          lp.reserved = 0;
-          a.log[pos] = lp;
-        }
-      }
-    }
-  }
-}
 
 template <int FORM>
 __device__ __forceinline__ bool prefilter(const PairScanArgs &a, const void *recs, uint32_t q, double px, double py,
@@ -285,12 +136,12 @@ __device__ __forceinline__ bool prefilter(const PairScanArgs &a, const void *rec
     const RecordF64 r = static_cast<const RecordF64 *>(recs)[q];
     const double dx = px - r.x, dy = py - r.y, dz = pz - r.z;
     oj = r.idx;
-    return !(dot3(dx, dy, dz, dx, dy, dz) > a.fp.cutoff2);
+    return !(dot3(dx, dy, dz, dx, dy, dz) > a.sink.fp.cutoff2);
   } else {
     const RecordF32 r = static_cast<const RecordF32 *>(recs)[q];
     const float dx = fx - r.x, dy = fy - r.y, dz = fz - r.z;
     oj = r.idx;
-    return !(dx * dx + dy * dy + dz * dz > a.fp.cutoff2_f32);
+    return !(dx * dx + dy * dy + dz * dz > a.sink.fp.cutoff2_f32);
   }
 }
 
@@ -298,7 +149,7 @@ template <int FORM, bool DIGEST>
 __global__ __launch_bounds__(256) void k_pair_scan(PairScanArgs a) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   ScanAcc acc;
-  if (p < a.o.n) {
+  if (p < a.sink.o.n) {
     double px = 0, py = 0, pz = 0;
     float fx = 0, fy = 0, fz = 0;
     uint32_t oi;
@@ -337,7 +188,7 @@ __global__ __launch_bounds__(256) void k_pair_scan(PairScanArgs a) {
 #pragma unroll 1
         for (; q < hi; ++q) {
           uint32_t oj;
-          if (prefilter<FORM>(a, a.rec, q, px, py, pz, fx, fy, fz, oj)) handle_candidate<DIGEST>(a, oi, oj, acc);
+          if (prefilter<FORM>(a, a.rec, q, px, py, pz, fx, fy, fz, oj)) handle_candidate<DIGEST>(a.sink, oi, oj, acc);
         }
         if (bz1 != 0u) break;
         lo = a.start[row];
@@ -345,26 +196,7 @@ __global__ __launch_bounds__(256) void k_pair_scan(PairScanArgs a) {
       }
     }
   }
-  // one atomic per wavefront for the counters
-  const unsigned long long cand = wave_reduce_sum(acc.cand);
-  if constexpr (DIGEST) {
-    const unsigned long long hit = wave_reduce_sum(acc.hit), ev = wave_reduce_sum(acc.ev);
-    const unsigned long long d0 = wave_reduce_sum(acc.d0), d1 = wave_reduce_sum(acc.d1);
-    const unsigned long long e0 = wave_reduce_sum(acc.e0), e1 = wave_reduce_sum(acc.e1);
-    if (lane_id() == 0) {
-      if (hit) {
-        atomicAdd(&a.counters->hits, hit);
-        atomicAdd(&a.counters->digest[0], d0);
-        atomicAdd(&a.counters->digest[1], d1);
-      }
-      if (ev) {
-        atomicAdd(&a.counters->evaded, ev);
-        atomicAdd(&a.counters->edigest[0], e0);
-        atomicAdd(&a.counters->edigest[1], e1);
-      }
-    }
-  }
-  if (lane_id() == 0 && cand) atomicAdd(&a.counters->candidates, cand);
+  flush_counters<DIGEST>(a.sink, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -412,16 +244,10 @@ hipError_t launch_pair_scan(int form, bool digest, const DeviceObjects &o, const
                             ScanCounters *counters, uint64_t iteration, hipStream_t s) {
   if (o.n == 0) return hipSuccess;
   PairScanArgs a;
-  a.o = o;
+  a.sink = PairSink{o, fp, log, log_cap, log_count, counters, iteration};
   a.g = g;
-  a.fp = fp;
   a.rec = rec;
   a.start = start;
-  a.log = log;
-  a.log_cap = log_cap;
-  a.log_count = log_count;
-  a.counters = counters;
-  a.iteration = iteration;
   const dim3 grid((unsigned)((o.n + 255) / 256));
   const bool f64 = (form == kFormFp64);
   if (f64 && !digest) hipLaunchKernelGGL((k_pair_scan<kFormFp64, false>), grid, dim3(256), 0, s, a);

@@ -42,6 +42,17 @@ struct GridParams {
   uint32_t nbuckets;     // Kx * Ky * Kz
 };
 
+// Plane-tile decomposition of the folded grid (tiles.hip): planes are the 2^bx
+// x-slices of 2^plane_bits buckets; objects are partitioned in tiles of tile_size.
+struct TileParams {
+  uint32_t tile_size;        // objects per partition tile
+  uint32_t ntiles;
+  uint32_t plane_bits;       // by + bz
+  uint32_t nplanes;          // 2^bx
+  uint32_t cap_items;        // records of one plane pair that fit the pair kernel's LDS
+  uint32_t lds_header_words; // LDS words before the record array in the pair kernel
+};
+
 // Sorted record of the plain FP64 form and of the split form: the position the
 // object is binned by, and its index in the object arrays.
 struct alignas(32) RecordF64 {

@@ -11,8 +11,8 @@ hipError_t upload_atmosphere();
 
 hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<double> *env64,
                             const SubstepEnv<float> *env32, int nsteps, unsigned mask, const double cdt[4],
-                            const double ddt[3], const GridParams *grid, uint32_t *count, uint32_t *slot,
-                            double burnup_r2, hipStream_t stream);
+                            const double ddt[3], const GridParams *grid, uint32_t *keys, uint32_t *count,
+                            uint32_t *slot, double burnup_r2, hipStream_t stream);
 
 hipError_t launch_bin_count(int form, const DeviceObjects &o, const GridParams &g, uint32_t *count, uint32_t *slot,
                             hipStream_t s);
@@ -25,6 +25,20 @@ hipError_t launch_pair_scan(int form, bool digest, const DeviceObjects &o, const
                             const FunctorParams &fp, const void *rec, const uint32_t *start,
                             LoggedPair *log, uint64_t log_cap, unsigned long long *log_count,
                             ScanCounters *counters, uint64_t iteration, hipStream_t s);
+
+// Plane-tile pipeline (tiles.hip)
+hipError_t launch_bin_keys(int form, const DeviceObjects &o, const GridParams &g, uint32_t *keys, hipStream_t s);
+hipError_t launch_tile_partition(int form, const DeviceObjects &o, const uint32_t *keys, const TileParams &tp,
+                                 void *items, uint32_t *tps, uint32_t *arena_top, hipStream_t s);
+hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, const GridParams &g,
+                             const TileParams &tp, const FunctorParams &fp, const void *items, const uint32_t *tps,
+                             void *scratch, uint32_t *arena_top, LoggedPair *log, uint64_t log_cap,
+                             unsigned long long *log_count, ScanCounters *counters, uint64_t iteration,
+                             hipStream_t s);
+uint32_t pair_tiles_header_words(const TileParams &tp);
+hipError_t configure_tile_kernels();
+constexpr uint32_t kPairTileLdsBytes = 160 * 1024;
+constexpr uint32_t kPartitionTile = 8192;
 
 // Search radius of the pair scan: the cutoff in the FP64 form; in the split form the
 // FP32 prefilter radius, inflated so that FP32 rounding of positions (< 2^-24 |x| per

@@ -31,7 +31,8 @@ struct PropagateArgs {
   unsigned mask;
   double cdt[4], ddt[3];
   GridParams grid;
-  uint32_t *count;  // non-null: binning count pass
+  uint32_t *keys;   // non-null: write the bucket key of the binning point
+  uint32_t *count;  // non-null: counting pass of the atomic counting sort
   uint32_t *slot;
   double burnup_r2;  // > 0: objects with |r|^2 below become dummies (deleteBurnUps)
 };
@@ -92,10 +93,13 @@ __global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
   a.o.vx[i] = vx;
   a.o.vy[i] = vy;
   a.o.vz[i] = vz;
-  if (a.count) {
+  if (a.keys || a.count) {
     binning_point<FORM != kFormFp64>(x, y, z);
     const CellInfo c = cell_of(x, y, z, a.grid);
-    a.slot[i] = atomicAdd(&a.count[c.bucket], 1u);
+    if (a.keys)
+      a.keys[i] = c.bucket;
+    else
+      a.slot[i] = atomicAdd(&a.count[c.bucket], 1u);
   }
 }
 
@@ -111,8 +115,8 @@ hipError_t upload_atmosphere() {
 
 hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<double> *env64,
                             const SubstepEnv<float> *env32, int nsteps, unsigned mask, const double cdt[4],
-                            const double ddt[3], const GridParams *grid, uint32_t *count, uint32_t *slot,
-                            double burnup_r2, hipStream_t stream) {
+                            const double ddt[3], const GridParams *grid, uint32_t *keys, uint32_t *count,
+                            uint32_t *slot, double burnup_r2, hipStream_t stream) {
   if (o.n == 0 || nsteps <= 0) return hipSuccess;
   PropagateArgs a;
   a.o = o;
@@ -123,6 +127,7 @@ hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<d
   for (int k = 0; k < 4; ++k) a.cdt[k] = cdt[k];
   for (int k = 0; k < 3; ++k) a.ddt[k] = ddt[k];
   if (grid) a.grid = *grid;
+  a.keys = keys;
   a.count = count;
   a.slot = slot;
   a.burnup_r2 = burnup_r2;

@@ -1,0 +1,445 @@
+// tiles.hip -- atomic-free binning and the LDS-tiled pair scan.
+//
+// Replaces AutoPas::iteratePairwise(&CollisionFunctor) over LinkedCells
+// (src/ladds/Simulation.cpp:89-92,211-222).  The folded grid of binning.h is viewed
+// as 2^bx x-planes of 2^pb buckets (pb = by + bz).  Per detection:
+//
+//   keys       bucket of every object's binning point (written by the propagation
+//              kernel, or by k_bin_keys when no propagation precedes the scan)
+//   partition  one workgroup per tile of consecutive objects: LDS histogram of the
+//              planes, LDS prefix, then every object's record is written into the
+//              tile's own region grouped by plane (contiguous, coalesced writes) and
+//              the tile's per-plane offsets are stored.  No global atomics.
+//   pairs      one workgroup per plane: gathers the plane's records and those of the
+//              next plane from every tile, counting-sorts both by bucket in LDS, and
+//              scans the 3x3x3 cells around each record from LDS (cells of side >= R,
+//              so a plane is one cell thick and holds half the objects of a 2R plane).
+//              Pairs inside the plane are taken from the lower sorted position, pairs
+//              with the next plane from the plane below, so every pair within the
+//              search radius is seen exactly once.  Candidates go through the exact
+//              FP64 predicate (pairs.h); hits are compacted per wavefront.
+//   A plane whose records exceed the LDS capacity is sorted into a global scratch
+//   arena instead (same code, global storage).
+#include <hip/hip_runtime.h>
+
+#include "binning.h"
+#include "launch.h"
+#include "pairs.h"
+
+namespace ladds {
+
+namespace {
+
+constexpr int kPartThreads = 1024;
+constexpr int kPartItems = kPartitionTile / kPartThreads;  // objects per thread in the partition
+constexpr int kPairThreads = 256;
+constexpr int kPairRegItems = 8;  // records per thread kept in registers between the two sort passes
+
+// In-place exclusive scan of a[0..n) in LDS by the whole block; returns the total.
+// `wsum` holds blockDim/64 + 1 entries.
+__device__ uint32_t block_exclusive_scan(uint32_t *a, uint32_t n, uint32_t *wsum) {
+  const uint32_t nt = blockDim.x, tid = threadIdx.x;
+  const uint32_t per = (n + nt - 1) / nt;
+  const uint32_t lo = min(n, tid * per), hi = min(n, lo + per);
+  uint32_t tot = 0;
+  for (uint32_t k = lo; k < hi; ++k) tot += a[k];
+  uint32_t inc = tot;
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(inc, off, 64);
+    if (lane >= (uint32_t)off) inc += t;
+  }
+  const uint32_t w = tid >> 6, nw = nt >> 6;
+  __syncthreads();  // a[] reads above are complete before anyone rewrites
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < nw; ++k) {
+      const uint32_t v = wsum[k];
+      wsum[k] = run;
+      run += v;
+    }
+    wsum[nw] = run;
+  }
+  __syncthreads();
+  uint32_t run = wsum[w] + inc - tot;
+  for (uint32_t k = lo; k < hi; ++k) {
+    const uint32_t v = a[k];
+    a[k] = run;
+    run += v;
+  }
+  const uint32_t total = wsum[nw];
+  __syncthreads();
+  return total;
+}
+
+template <typename REC>
+__device__ __forceinline__ void binning_point_of(const REC &r, double &x, double &y, double &z) {
+  x = (double)r.x;
+  y = (double)r.y;
+  z = (double)r.z;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// keys and partition
+// ---------------------------------------------------------------------------
+template <bool F32>
+__global__ __launch_bounds__(256) void k_bin_keys(const double *rx, const double *ry, const double *rz, size_t n,
+                                                  GridParams g, uint32_t *keys) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = rx[i], y = ry[i], z = rz[i];
+  binning_point<F32>(x, y, z);
+  keys[i] = cell_of(x, y, z, g).bucket;
+}
+
+template <typename REC>
+__global__ __launch_bounds__(kPartThreads) void k_tile_partition(const double *rx, const double *ry,
+                                                                  const double *rz, size_t n, const uint32_t *keys,
+                                                                  TileParams tp, REC *items, uint32_t *tps,
+                                                                  uint32_t *arena_top) {
+  extern __shared__ uint32_t lds[];
+  uint32_t *cnt = lds;                // nplanes
+  uint32_t *wsum = lds + tp.nplanes;  // kPartThreads / 64 + 1
+  const uint32_t P = tp.nplanes, pb = tp.plane_bits;
+  const size_t base = (size_t)blockIdx.x * tp.tile_size;
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) cnt[p] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *arena_top = 0;
+  __syncthreads();
+  // every thread owns kPartItems objects of the tile; their loads are issued together
+  uint32_t key[kPartItems];
+#pragma unroll
+  for (int k = 0; k < kPartItems; ++k) {
+    const size_t i = base + threadIdx.x + (size_t)k * kPartThreads;
+    key[k] = (i < n) ? keys[i] : ~0u;
+  }
+#pragma unroll
+  for (int k = 0; k < kPartItems; ++k)
+    if (key[k] != ~0u) atomicAdd(&cnt[key[k] >> pb], 1u);
+  __syncthreads();
+  const uint32_t total = block_exclusive_scan(cnt, P, wsum);
+  uint32_t *row = tps + (size_t)blockIdx.x * (P + 1);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) row[p] = cnt[p];
+  if (threadIdx.x == 0) row[P] = total;
+  __syncthreads();
+  constexpr bool f32 = sizeof(REC) == sizeof(RecordF32);
+  double px[kPartItems], py[kPartItems], pz[kPartItems];
+#pragma unroll
+  for (int k = 0; k < kPartItems; ++k) {
+    const size_t i = base + threadIdx.x + (size_t)k * kPartThreads;
+    if (key[k] != ~0u) {
+      px[k] = rx[i];
+      py[k] = ry[i];
+      pz[k] = rz[i];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPartItems; ++k) {
+    if (key[k] == ~0u) continue;
+    const size_t i = base + threadIdx.x + (size_t)k * kPartThreads;
+    const uint32_t pos = atomicAdd(&cnt[key[k] >> pb], 1u);
+    double x = px[k], y = py[k], z = pz[k];
+    binning_point<f32>(x, y, z);
+    REC r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    r.idx = (uint32_t)i;
+    if constexpr (!f32) r.pad = 0;
+    items[base + pos] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pair scan over plane tiles
+// ---------------------------------------------------------------------------
+struct PairTileArgs {
+  PairSink sink;
+  GridParams g;
+  TileParams tp;
+  const void *items;
+  const uint32_t *tps;
+  void *scratch;
+  uint32_t *arena_top;
+};
+
+template <int FORM>
+using RecOf = typename std::conditional<FORM == kFormFp64, RecordF64, RecordF32>::type;
+
+template <int FORM>
+__device__ __forceinline__ bool within(const RecOf<FORM> &a, const RecOf<FORM> &b, const FunctorParams &fp) {
+  if constexpr (FORM == kFormFp64) {
+    const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return !(exact_dot3(dx, dy, dz, dx, dy, dz) > fp.cutoff2);
+  } else {
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return !(dx * dx + dy * dy + dz * dz > fp.cutoff2_f32);
+  }
+}
+
+// Scan one record range [lo, hi) of a sorted plane array.
+template <int FORM, bool DIGEST>
+__device__ __forceinline__ void scan_range(const PairTileArgs &a, const RecOf<FORM> &me, const RecOf<FORM> *arr,
+                                           uint32_t lo, uint32_t hi, ScanAcc &acc) {
+#pragma unroll 1
+  for (uint32_t q = lo; q < hi; ++q) {
+    const RecOf<FORM> other = arr[q];
+    if (within<FORM>(me, other, a.sink.fp)) handle_candidate<DIGEST>(a.sink, me.idx, other.idx, acc);
+  }
+}
+
+// Bucket ranges of the sorted plane arrays: cur[b] = end of bucket b after placement.
+__device__ __forceinline__ uint32_t bucket_begin(const uint32_t *cur, uint32_t b) { return b ? cur[b - 1] : 0u; }
+
+// Scan the up to three contiguous bucket runs covering z in {fz-1, fz, fz+1} (mod Kz) of one
+// row of a sorted plane array; records before `from` are skipped.
+template <int FORM, bool DIGEST>
+__device__ __forceinline__ void scan_row(const PairTileArgs &a, const RecOf<FORM> &me, const RecOf<FORM> *arr,
+                                         const uint32_t *cur, uint32_t row, uint32_t fz, uint32_t mz, uint32_t from,
+                                         ScanAcc &acc) {
+  if (fz != 0u && fz != mz) {
+    scan_range<FORM, DIGEST>(a, me, arr, max(bucket_begin(cur, row | (fz - 1u)), from), cur[row | (fz + 1u)], acc);
+  } else if (fz == 0u) {
+    scan_range<FORM, DIGEST>(a, me, arr, max(bucket_begin(cur, row), from), cur[row | 1u], acc);
+    scan_range<FORM, DIGEST>(a, me, arr, max(bucket_begin(cur, row | mz), from), cur[row | mz], acc);
+  } else {  // fz == mz
+    scan_range<FORM, DIGEST>(a, me, arr, max(bucket_begin(cur, row | (mz - 1u)), from), cur[row | mz], acc);
+    scan_range<FORM, DIGEST>(a, me, arr, max(bucket_begin(cur, row), from), cur[row], acc);
+  }
+}
+
+// Cells have side >= the search radius R, so partners within R lie in the 3x3x3 cells
+// around an object.  Within its own plane a pair is taken from the lower sorted
+// position; pairs with the next plane are all taken here (the next plane's own scan
+// never looks back), pairs with the previous plane by the previous plane's workgroup.
+template <int FORM, bool DIGEST>
+__device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FORM> *own, const RecOf<FORM> *halo,
+                                           const uint32_t *cur_own, const uint32_t *cur_halo, uint32_t n_own,
+                                           ScanAcc &acc) {
+  const GridParams &g = a.g;
+  const uint32_t my = (1u << g.by) - 1u, mz = (1u << g.bz) - 1u;
+  const uint32_t sy = g.bz;
+#pragma unroll 1
+  for (uint32_t s = threadIdx.x; s < n_own; s += blockDim.x) {
+    const RecOf<FORM> me = own[s];
+    double x, y, z;
+    binning_point_of(me, x, y, z);
+    const uint32_t lb = cell_of(x, y, z, g).bucket;
+    const uint32_t fy = (lb >> sy) & my, fz = lb & mz;
+#pragma unroll 1
+    for (uint32_t dy = 0; dy < 3; ++dy) {
+      const uint32_t row = ((fy + dy - 1u) & my) << sy;
+      scan_row<FORM, DIGEST>(a, me, own, cur_own, row, fz, mz, s + 1u, acc);
+      scan_row<FORM, DIGEST>(a, me, halo, cur_halo, row, fz, mz, 0u, acc);
+    }
+  }
+}
+
+template <int FORM, bool DIGEST>
+__global__ __launch_bounds__(kPairThreads) void k_pair_tiles(PairTileArgs a) {
+  using REC = RecOf<FORM>;
+  extern __shared__ uint32_t lds[];
+  const TileParams &tp = a.tp;
+  const uint32_t T = tp.ntiles, P = tp.nplanes, L = 1u << tp.plane_bits;
+  const uint32_t plane = blockIdx.x, next = (plane + 1u) & (P - 1u);  // P >= 4
+  // LDS layout
+  uint32_t *seg_own = lds;               // T + 1 prefix of segment lengths
+  uint32_t *seg_next = seg_own + (T + 1);
+  uint32_t *src_own = seg_next + (T + 1);  // T source offsets
+  uint32_t *src_next = src_own + T;
+  uint32_t *cur_own = src_next + T;        // L bucket cursors
+  uint32_t *cur_next = cur_own + L;
+  uint32_t *wsum = cur_next + L;           // kPairThreads / 64 + 1
+  uint32_t *misc = wsum + (kPairThreads / 64 + 1);  // [0] scratch offset
+  REC *lds_items = reinterpret_cast<REC *>(lds + tp.lds_header_words);
+
+  const REC *items = static_cast<const REC *>(a.items);
+  for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+    const uint32_t *row = a.tps + (size_t)t * (P + 1);
+    const uint32_t o0 = row[plane], o1 = row[plane + 1];
+    const uint32_t n0 = row[next], n1 = row[next + 1];
+    seg_own[t] = o1 - o0;
+    seg_next[t] = n1 - n0;
+    src_own[t] = t * tp.tile_size + o0;
+    src_next[t] = t * tp.tile_size + n0;
+  }
+  for (uint32_t b = threadIdx.x; b < L; b += blockDim.x) {
+    cur_own[b] = 0;
+    cur_next[b] = 0;
+  }
+  __syncthreads();
+  const uint32_t n_own = block_exclusive_scan(seg_own, T, wsum);
+  const uint32_t n_next = block_exclusive_scan(seg_next, T, wsum);
+  if (threadIdx.x == 0) {
+    seg_own[T] = n_own;
+    seg_next[T] = n_next;
+  }
+  ScanAcc acc;
+  if (n_own != 0u) {
+    const uint32_t total = n_own + n_next;
+    const bool in_lds = total <= tp.cap_items;
+    if (!in_lds && threadIdx.x == 0) misc[0] = atomicAdd(a.arena_top, total);
+    __syncthreads();
+    REC *sorted = in_lds ? lds_items : static_cast<REC *>(a.scratch) + misc[0];
+    REC *own = sorted, *halo = sorted + n_own;
+    const uint32_t mloc = L - 1u;
+    // j enumerates own then next records; loc() finds record j in its tile segment
+    auto loc = [&](uint32_t j) -> uint32_t {
+      const bool is_own = j < n_own;
+      const uint32_t k = is_own ? j : j - n_own;
+      const uint32_t *seg = is_own ? seg_own : seg_next;
+      uint32_t lo = 0, hi = T;  // segment t with seg[t] <= k < seg[t+1]
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (seg[mid] <= k)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      return (is_own ? src_own : src_next)[lo] + (k - seg[lo]);
+    };
+    auto local_bucket = [&](const REC &r) -> uint32_t {
+      double x, y, z;
+      binning_point_of(r, x, y, z);
+      return cell_of(x, y, z, a.g).bucket & mloc;
+    };
+    if (total <= (uint32_t)(kPairRegItems * kPairThreads)) {
+      // records stay in registers between the histogram and the placement pass
+      REC reg[kPairRegItems];
+      uint32_t lb[kPairRegItems];
+#pragma unroll
+      for (int k = 0; k < kPairRegItems; ++k) {
+        const uint32_t j = threadIdx.x + k * kPairThreads;
+        if (j < total) reg[k] = items[loc(j)];
+      }
+#pragma unroll
+      for (int k = 0; k < kPairRegItems; ++k) {
+        const uint32_t j = threadIdx.x + k * kPairThreads;
+        if (j < total) {
+          lb[k] = local_bucket(reg[k]);
+          atomicAdd(&(j < n_own ? cur_own : cur_next)[lb[k]], 1u);
+        }
+      }
+      __syncthreads();
+      block_exclusive_scan(cur_own, L, wsum);
+      block_exclusive_scan(cur_next, L, wsum);
+#pragma unroll
+      for (int k = 0; k < kPairRegItems; ++k) {
+        const uint32_t j = threadIdx.x + k * kPairThreads;
+        if (j < total) {
+          const bool is_own = j < n_own;
+          const uint32_t pos = atomicAdd(&(is_own ? cur_own : cur_next)[lb[k]], 1u);
+          (is_own ? own : halo)[pos] = reg[k];
+        }
+      }
+      __syncthreads();
+    } else {
+      // pass 1: bucket histogram; pass 2: placement (records read twice)
+#pragma unroll 1
+      for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll 1
+        for (uint32_t j = threadIdx.x; j < total; j += blockDim.x) {
+          const bool is_own = j < n_own;
+          const REC r = items[loc(j)];
+          uint32_t *cur = is_own ? cur_own : cur_next;
+          const uint32_t b = local_bucket(r);
+          if (pass == 0) {
+            atomicAdd(&cur[b], 1u);
+          } else {
+            const uint32_t pos = atomicAdd(&cur[b], 1u);
+            (is_own ? own : halo)[pos] = r;
+          }
+        }
+        __syncthreads();
+        if (pass == 0) {
+          block_exclusive_scan(cur_own, L, wsum);
+          block_exclusive_scan(cur_next, L, wsum);
+        }
+      }
+    }
+    // cur[b] now holds the end of bucket b
+    if (in_lds)
+      scan_plane<FORM, DIGEST>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, acc);
+    else
+      scan_plane<FORM, DIGEST>(a, own, halo, cur_own, cur_next, n_own, acc);
+  }
+  flush_counters<DIGEST>(a.sink, acc);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_bin_keys(int form, const DeviceObjects &o, const GridParams &g, uint32_t *keys, hipStream_t s) {
+  if (o.n == 0) return hipSuccess;
+  const dim3 grid((unsigned)((o.n + 255) / 256));
+  if (form == kFormFp64)
+    hipLaunchKernelGGL(k_bin_keys<false>, grid, dim3(256), 0, s, o.rx, o.ry, o.rz, o.n, g, keys);
+  else
+    hipLaunchKernelGGL(k_bin_keys<true>, grid, dim3(256), 0, s, o.rx, o.ry, o.rz, o.n, g, keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_partition(int form, const DeviceObjects &o, const uint32_t *keys, const TileParams &tp,
+                                 void *items, uint32_t *tps, uint32_t *arena_top, hipStream_t s) {
+  if (o.n == 0) return hipSuccess;
+  const size_t shm = sizeof(uint32_t) * (tp.nplanes + kPartThreads / 64 + 1);
+  if (form == kFormFp64)
+    hipLaunchKernelGGL(k_tile_partition<RecordF64>, dim3(tp.ntiles), dim3(kPartThreads), shm, s, o.rx, o.ry, o.rz,
+                       o.n, keys, tp, static_cast<RecordF64 *>(items), tps, arena_top);
+  else
+    hipLaunchKernelGGL(k_tile_partition<RecordF32>, dim3(tp.ntiles), dim3(kPartThreads), shm, s, o.rx, o.ry, o.rz,
+                       o.n, keys, tp, static_cast<RecordF32 *>(items), tps, arena_top);
+  return hipGetLastError();
+}
+
+hipError_t configure_tile_kernels() {
+  const int max_lds = 160 * 1024;
+  hipError_t e = hipSuccess;
+  auto set = [&](const void *f) {
+    if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+  };
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, true>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, true>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32>));
+  return e;
+}
+
+uint32_t pair_tiles_header_words(const TileParams &tp) {
+  const uint32_t L = 1u << tp.plane_bits;
+  uint32_t w = 4 * tp.ntiles + 2 + 2 * L + kPairThreads / 64 + 1 + 4;
+  return (w + 7u) & ~7u;  // 32-byte alignment of the record array
+}
+
+hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, const GridParams &g,
+                             const TileParams &tp, const FunctorParams &fp, const void *items, const uint32_t *tps,
+                             void *scratch, uint32_t *arena_top, LoggedPair *log, uint64_t log_cap,
+                             unsigned long long *log_count, ScanCounters *counters, uint64_t iteration,
+                             hipStream_t s) {
+  if (o.n == 0) return hipSuccess;
+  PairTileArgs a;
+  a.sink = PairSink{o, fp, log, log_cap, log_count, counters, iteration};
+  a.g = g;
+  a.tp = tp;
+  a.items = items;
+  a.tps = tps;
+  a.scratch = scratch;
+  a.arena_top = arena_top;
+  const size_t rec = (form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
+  const size_t shm = sizeof(uint32_t) * tp.lds_header_words + rec * tp.cap_items;
+  const dim3 grid(tp.nplanes), block(kPairThreads);
+  const bool f64 = (form == kFormFp64);
+  if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false>), grid, block, shm, s, a);
+  if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true>), grid, block, shm, s, a);
+  if (!f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, false>), grid, block, shm, s, a);
+  if (!f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, true>), grid, block, shm, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace ladds

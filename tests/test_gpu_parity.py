@@ -17,8 +17,16 @@ pytestmark = pytest.mark.gpu
 UMAX = np.iinfo(np.uint64).max
 
 
+VARIANTS = [1, 2]  # pair-scan pipelines: atomic counting sort, plane tiles
+
+
 def engine(**kw):
     return la.Engine(la.EngineConfig(**kw))
+
+
+@pytest.fixture(params=VARIANTS, ids=["atomic_sort", "plane_tiles"])
+def variant(request):
+    return request.param
 
 
 def _rel_err(r_gpu, r_ref):
@@ -136,10 +144,10 @@ def _cloud(n, seed, box=200.0, centre=(6000.0, 3000.0, 1500.0), vel_sigma=5.0):
 
 @pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
 @pytest.mark.parametrize("mode,thr", [(la.THRESHOLD_RADIUS_SCALED, 1.0), (la.THRESHOLD_ABSOLUTE, 0.6)])
-def test_conjunctions_bit_identical(precision, mode, thr):
+def test_conjunctions_bit_identical(precision, mode, thr, variant):
     pop = _cloud(20000, 11)
     cutoff, dt = 2.0, 10.0
-    with engine(dt=dt, cutoff_km=cutoff, precision=precision, threshold_mode=mode, threshold_km=thr,
+    with engine(scan_variant=variant, dt=dt, cutoff_km=cutoff, precision=precision, threshold_mode=mode, threshold_km=thr,
                 collision_distance_factor=1.5, min_detection_radius_m=300.0, evasion_cutoff_km=0.8) as eng:
         eng.set_objects(pop)
         hits, ev = eng.conjunctions(dt)
@@ -150,11 +158,11 @@ def test_conjunctions_bit_identical(precision, mode, thr):
     _pair_equal(ol.sort_pairs(ev), oe)
 
 
-def test_conjunctions_after_propagation_match_bruteforce():
+def test_conjunctions_after_propagation_match_bruteforce(variant):
     """LEO population with planted pairs: GPU scan of the GPU state == oracle brute force.
     After 3 steps the co-orbital half of the 64 planted pairs is still within 1 km."""
     pop = la.generate_population(0, 0x1ADD5, 4096, planted_pairs=64)
-    with engine(dt=10.0, components=la.KEP | la.J2, cutoff_km=10.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
+    with engine(scan_variant=variant, dt=10.0, components=la.KEP | la.J2, cutoff_km=10.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
                 threshold_km=1.0) as eng:
         eng.set_objects(pop)
         eng.propagate(3)
@@ -168,12 +176,12 @@ def test_conjunctions_after_propagation_match_bruteforce():
 
 
 @pytest.mark.parametrize("name", ["three_particles", "mix_activity_states"])
-def test_reference_known_answers_on_gpu(golden_cases, name):
+def test_reference_known_answers_on_gpu(golden_cases, name, variant):
     case = golden_cases[name]
     f = case["functor"]
     parts = case["particles"]
     n = len(parts)
-    with engine(dt=f["dt"], cutoff_km=f["cutoff"], collision_distance_factor=f["cdf"],
+    with engine(scan_variant=variant, dt=f["dt"], cutoff_km=f["cutoff"], collision_distance_factor=f["cdf"],
                 min_detection_radius_m=f["min_det"], evasion_cutoff_km=f["evasion"]) as eng:
         eng.set_objects(rx=[p["r"][0] for p in parts], ry=[p["r"][1] for p in parts],
                         rz=[p["r"][2] for p in parts], vx=[p["v"][0] for p in parts],
@@ -191,11 +199,11 @@ def test_reference_known_answers_on_gpu(golden_cases, name):
         assert n == 8 and np.all(np.isnan(hits["d2"]))
 
 
-def test_linear_interpolation_vectors_on_gpu(golden_cases):
+def test_linear_interpolation_vectors_on_gpu(golden_cases, variant):
     case = golden_cases["linear_interpolation"]
     f = case["functor"]
     for vec in case["vectors"]:
-        with engine(dt=vec["dt"], cutoff_km=f["cutoff"], collision_distance_factor=f["cdf"],
+        with engine(scan_variant=variant, dt=vec["dt"], cutoff_km=f["cutoff"], collision_distance_factor=f["cdf"],
                     min_detection_radius_m=f["min_det"], evasion_cutoff_km=f["evasion"]) as eng:
             eng.set_objects(rx=[vec["x1"][0], vec["x2"][0]], ry=[vec["x1"][1], vec["x2"][1]],
                             rz=[vec["x1"][2], vec["x2"][2]], vx=[vec["v1"][0], vec["v2"][0]],
@@ -206,9 +214,9 @@ def test_linear_interpolation_vectors_on_gpu(golden_cases):
         assert float(hits[0]["d2"]) == vec["d2"] and list(hits[0]["mid"]) == vec["mid"]
 
 
-def test_digest_matches_oracle():
+def test_digest_matches_oracle(variant):
     pop = _cloud(20000, 12)
-    with engine(dt=10.0, cutoff_km=2.0, precision=la.SPLIT, min_detection_radius_m=300.0) as eng:
+    with engine(scan_variant=variant, dt=10.0, cutoff_km=2.0, precision=la.SPLIT, min_detection_radius_m=300.0) as eng:
         eng.set_objects(pop)
         n, d, ne, e = eng.conjunction_digest(10.0)
         hits, ev = eng.conjunctions(10.0)
@@ -219,13 +227,13 @@ def test_digest_matches_oracle():
     assert hits.shape[0] == n
 
 
-def test_dense_cell_all_pairs():
+def test_dense_cell_all_pairs(variant):
     """1,000 objects inside 50 m: every pair is scanned once (499,500 candidates)."""
     n = 1000
     rng = np.random.default_rng(3)
     r = np.array([7000.0, 0.0, 0.0]) + rng.uniform(-0.025, 0.025, (n, 3))
     v = np.array([0.0, 7.5, 0.0]) + rng.normal(0, 1e-3, (n, 3))
-    with engine(dt=10.0, cutoff_km=5.0, threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=0.5) as eng:
+    with engine(scan_variant=variant, dt=10.0, cutoff_km=5.0, threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=0.5) as eng:
         eng.set_objects(rx=r[:, 0], ry=r[:, 1], rz=r[:, 2], vx=v[:, 0], vy=v[:, 1], vz=v[:, 2],
                         radius_m=np.full(n, 0.1))
         hits, _ = eng.conjunctions()
@@ -237,7 +245,7 @@ def test_dense_cell_all_pairs():
     assert dig == ol.digest(oh)
 
 
-def test_cell_boundaries_and_tiny_populations():
+def test_cell_boundaries_and_tiny_populations(variant):
     cutoff = 1.0
     cell = 2.0 * cutoff * (1 + 1e-9)
     # objects exactly on and around cell faces, within and just outside the cutoff
@@ -248,7 +256,7 @@ def test_cell_boundaries_and_tiny_populations():
     r = base + offs
     v = np.zeros_like(r) + np.array([0.0, 7.0, 0.0]) + np.arange(len(r))[:, None] * 1e-3
     n = r.shape[0]
-    with engine(dt=10.0, cutoff_km=cutoff, collision_distance_factor=1e6) as eng:
+    with engine(scan_variant=variant, dt=10.0, cutoff_km=cutoff, collision_distance_factor=1e6) as eng:
         eng.set_objects(rx=r[:, 0], ry=r[:, 1], rz=r[:, 2], vx=v[:, 0], vy=v[:, 1], vz=v[:, 2],
                         radius_m=np.full(n, 1.0))
         hits, _ = eng.conjunctions()
@@ -268,11 +276,11 @@ def test_cell_boundaries_and_tiny_populations():
 # ---------------------------------------------------------------------------
 # the simulation loop
 # ---------------------------------------------------------------------------
-def test_simulate_matches_oracle_step_by_step():
+def test_simulate_matches_oracle_step_by_step(variant):
     """integrate + detection every step (tpcd = 1): per-iteration id pair sets equal the oracle's."""
     pop = la.generate_population(0, 0x1ADD5, 8192, planted_pairs=48)
     mask, dt, steps = la.KEP | la.J2, 10.0, 12
-    with engine(dt=dt, components=mask, cutoff_km=10.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
+    with engine(scan_variant=variant, dt=dt, components=mask, cutoff_km=10.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
                 threshold_km=1.0) as eng:
         eng.set_objects(pop)
         stats = eng.simulate(steps, first_iteration=0, tpcd=1)
@@ -292,23 +300,23 @@ def test_simulate_matches_oracle_step_by_step():
     assert total >= 48
 
 
-def test_timesteps_per_collision_detection():
+def test_timesteps_per_collision_detection(variant):
     """SimulationTest.cpp:144-193: head-on pair, tpcd = 2 -> 0 / 0 / 1 conjunctions."""
     re = 6378.1363
     alt = re + 150.0
     for iterations, expected in ((1, 0), (2, 0), (3, 1)):
-        with engine(dt=1.0, components=la.KEP, cutoff_km=80.0) as eng:
+        with engine(scan_variant=variant, dt=1.0, components=la.KEP, cutoff_km=80.0) as eng:
             eng.set_objects(rx=[alt + 100.0, alt + 100.0], ry=[20.0, -20.0], rz=[0.0, 0.0], vx=[0.0, 0.0],
                             vy=[-10.0, 10.0], vz=[0.0, 0.0], radius_m=[1.0, 1.0], id=[0, 1])
             stats = eng.simulate(iterations, first_iteration=0, tpcd=2, min_altitude_km=150.0)
         assert stats["collisions"] == expected, iterations
 
 
-def test_burn_up():
+def test_burn_up(variant):
     """SimulationTest.cpp:102-136: the object heading to Earth burns up after one iteration."""
     re = 6378.1363
     crit = re + 150.0
-    with engine(dt=1.0, components=la.KEP, cutoff_km=80.0) as eng:
+    with engine(scan_variant=variant, dt=1.0, components=la.KEP, cutoff_km=80.0) as eng:
         eng.set_objects(rx=[crit + 1.0, 0.0], ry=[0.0, crit + 1.0], rz=[0.0, 0.0], vx=[-10.0, 0.0],
                         vy=[0.0, 10.0], vz=[0.0, 0.0], radius_m=[1.0, 1.0], id=[0, 1])
         stats = eng.simulate(1, first_iteration=0, tpcd=1, min_altitude_km=150.0)
@@ -334,11 +342,49 @@ def test_delete_burnups_compacts_in_order():
     assert np.array_equal(rg, r[keep])
 
 
-def test_log_overflow_is_reported():
+def test_log_overflow_is_reported(variant):
     pop = _cloud(5000, 13, box=20.0, vel_sigma=1e-3)
-    with engine(dt=10.0, cutoff_km=2.0, collision_distance_factor=1e6, log_capacity=1024) as eng:
+    with engine(scan_variant=variant, dt=10.0, cutoff_km=2.0, collision_distance_factor=1e6, log_capacity=1024) as eng:
         eng.set_objects(pop)
         stats = eng.simulate(1, tpcd=1)
         assert stats["collisions"] + stats["evasions"] > 1024
         with pytest.raises(la.LaddsError):
             eng.fetch_log()
+
+
+def test_plane_overflow_uses_scratch(variant):
+    """6,000 objects inside 2 km: one plane far beyond the LDS capacity of the tile kernel."""
+    n = 6000
+    rng = np.random.default_rng(5)
+    r = np.array([7000.0, 100.0, -50.0]) + rng.uniform(-1.0, 1.0, (n, 3))
+    v = np.array([0.0, 7.5, 0.1]) + rng.normal(0, 0.05, (n, 3))
+    with engine(scan_variant=variant, dt=10.0, cutoff_km=5.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
+                threshold_km=0.02) as eng:
+        eng.set_objects(rx=r[:, 0], ry=r[:, 1], rz=r[:, 2], vx=v[:, 0], vy=v[:, 1], vz=v[:, 2],
+                        radius_m=np.full(n, 0.1))
+        hits, _ = eng.conjunctions()
+        cnt, dig, _, _ = eng.conjunction_digest()
+    objs = ol.ObjectSet(r, v, np.full(n, 0.1))
+    oh, _ = ol.pairs(objs, ol.functor_cfg(5.0, 10.0, threshold_mode=1, threshold_km=0.02), method="grid")
+    assert oh.shape[0] > 10
+    _pair_equal(ol.sort_pairs(hits), oh)
+    assert cnt == oh.shape[0] and dig == ol.digest(oh)
+
+
+def test_variants_agree_on_leo_population():
+    """Both pipelines and both forms report the same pairs after a few steps of the full model."""
+    pop = la.generate_population(1, 99, 60000, planted_pairs=200)
+    got = {}
+    for v in VARIANTS:
+        for prec in (la.FP64, la.SPLIT):
+            with engine(scan_variant=v, precision=prec, dt=10.0, components=la.ALL_COMPONENTS, cutoff_km=10.0,
+                        threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0) as eng:
+                eng.set_objects(pop)
+                st = eng.simulate(5, tpcd=1, min_altitude_km=150.0)
+                log = eng.fetch_log()
+            keys = sorted(zip(log["iteration"].tolist(), log["pair"]["id_a"].tolist(), log["pair"]["id_b"].tolist()))
+            got[(v, prec)] = (keys, st["candidates"] > 0)
+    ref = got[(1, la.FP64)][0]
+    assert len(ref) >= 100
+    for k, (keys, _) in got.items():
+        assert keys == ref, k
