@@ -157,6 +157,13 @@ LADDS_API void ladds_config_default(ladds_config *cfg);
 LADDS_API int ladds_create(const ladds_config *cfg, ladds_ctx **out);
 LADDS_API int ladds_destroy(ladds_ctx *ctx);
 LADDS_API const char *ladds_last_error(const ladds_ctx *ctx);
+/* Update the parameters the integrator, accumulator and CollisionFunctor own (dt, t0,
+ * components, functor thresholds, pair output, log capacity) -- the reference sets them in
+ * the constructors of YoshidaIntegrator (Simulation.cpp:145), AccelerationAccumulator
+ * (:142-143) and CollisionFunctor (CollisionFunctor.cpp:18-30).  Changing t0 or dt restarts
+ * the step count at t0.  Fields fixed at ladds_create (precision, cutoff_km, cell_size_km,
+ * scan_variant, device) must be unchanged once objects are set (LADDS_E_STATE otherwise). */
+LADDS_API int ladds_configure(ladds_ctx *ctx, const ladds_config *cfg);
 
 /* AutoPas::addParticle for a whole population (SatelliteLoader.cpp:17-84). Replaces all objects. */
 LADDS_API int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *objs);

@@ -174,6 +174,15 @@ class Engine:
             msg = lib.ladds_last_error(self._h)
             raise LaddsError(rc, f"{what}: {msg.decode() if msg else ''}")
 
+    def configure(self, **changes):
+        """Update dt, t0, components and the CollisionFunctor parameters (ladds_configure)."""
+        for k, v in changes.items():
+            if not hasattr(self.config, k):
+                raise AttributeError(k)
+            setattr(self.config, k, v)
+        self._c = self.config.to_c()
+        self._check(lib.ladds_configure(self._h, C.byref(self._c)), "configure")
+
     # -- state ----------------------------------------------------------
     def set_objects(self, pop: Population | None = None, **arrays):
         """``AutoPas::addParticle`` for a whole population (replaces all objects)."""

@@ -117,6 +117,7 @@ _SIGNATURES = {
     "ladds_create": (C.c_int, [C.POINTER(Config), C.POINTER(_ctx)]),
     "ladds_destroy": (C.c_int, [_ctx]),
     "ladds_last_error": (C.c_char_p, [_ctx]),
+    "ladds_configure": (C.c_int, [_ctx, C.POINTER(Config)]),
     "ladds_set_objects": (C.c_int, [_ctx, C.POINTER(Objects)]),
     "ladds_num_objects": (C.c_size_t, [_ctx]),
     "ladds_get_state": (C.c_int, [_ctx, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -144,7 +145,7 @@ def load(path: Path | str = LIB_PATH) -> C.CDLL:
     path = Path(path)
     if not path.exists():
         raise ImportError(
-            f"{path} is missing: build the gfx950 engine first (python -m ladds_amd.build)")
+            f"{path} is missing: build the gfx950 engine first (python ladds_amd/build.py)")
     lib = C.CDLL(str(path))
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)

@@ -5,7 +5,8 @@ gfx950 only, then linked into one shared library whose exported symbols are
 exactly the C ABI of ``include/ladds_amd.h``.  Packed FP32 instructions are
 disabled (``-Xclang -target-feature -Xclang -packed-fp32-ops``).
 
-Run ``python -m ladds_amd.build`` (or ``__graft_entry__.build()``).
+Run ``python ladds_amd/build.py`` (or ``__graft_entry__.build()``); it does not import
+the package, whose import loads the library this script builds.
 """
 from __future__ import annotations
 

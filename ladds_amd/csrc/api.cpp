@@ -539,6 +539,31 @@ int ladds_destroy(ladds_ctx *ctx) {
 
 const char *ladds_last_error(const ladds_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int ladds_configure(ladds_ctx *ctx, const ladds_config *cfg) {
+  if (!ctx || !cfg) return LADDS_E_INVALID;
+  if (!(cfg->dt > 0.0) || cfg->threshold_mode < 0 || cfg->threshold_mode > 1 || cfg->pair_output < 0 ||
+      cfg->pair_output > 1)
+    return ctx->fail(LADDS_E_INVALID, "invalid dt, threshold_mode or pair_output");
+  const ladds_config &old = ctx->cfg;
+  const bool fixed_changed = cfg->precision != old.precision || cfg->cutoff_km != old.cutoff_km ||
+                             cfg->cell_size_km != old.cell_size_km || cfg->scan_variant != old.scan_variant ||
+                             cfg->device != old.device;
+  if (fixed_changed) {
+    if (ctx->n > 0)
+      return ctx->fail(LADDS_E_STATE, "precision, cutoff, cell size, scan variant and device are fixed once objects are set");
+    if (!(cfg->cutoff_km > 0.0) || cfg->precision < 0 || cfg->precision > 2 || cfg->scan_variant < 0 ||
+        cfg->scan_variant > 2 || cfg->device != old.device)
+      return ctx->fail(LADDS_E_INVALID, "invalid precision, cutoff or scan variant (device is fixed)");
+    ctx->form = cfg->precision;
+    ctx->variant = (cfg->scan_variant == kVariantAtomicSort) ? kVariantAtomicSort : kVariantPlaneTiles;
+  }
+  if (cfg->t0 != old.t0 || cfg->dt != old.dt) ctx->step_count = 0;
+  const uint64_t log_cap = cfg->log_capacity;
+  ctx->cfg = *cfg;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return ensure_log(ctx, std::max<uint64_t>(log_cap, 1024));
+}
+
 int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
   if (!ctx || !o) return LADDS_E_INVALID;
   if (o->n > 0 && (!o->rx || !o->ry || !o->rz || !o->vx || !o->vy || !o->vz || !o->radius_m))
