@@ -22,6 +22,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libladds_amd.so"
+ADAPTER_CHECK = ROOT / "tests" / "cpp" / "adapter_check"
 ARCH = "gfx950"
 
 SOURCES = [
@@ -76,7 +77,22 @@ def build(verbose: bool = False) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_adapter_check(verbose)
     return LIB
+
+
+def build_adapter_check(verbose: bool = False) -> Path:
+    """Host-only C++ program that exercises the C++ adapter (include/ladds_amd.hpp)
+    against the library; run by tests/test_cpp_adapter.py."""
+    src = ADAPTER_CHECK.with_suffix(".cpp")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", str(src), "-o",
+           str(ADAPTER_CHECK), f"-L{PKG}", "-lladds_amd", "-Wl,-rpath,$ORIGIN/../../ladds_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"adapter check build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {ADAPTER_CHECK}")
+    return ADAPTER_CHECK
 
 
 if __name__ == "__main__":

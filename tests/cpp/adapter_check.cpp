@@ -1,0 +1,120 @@
+// Drives the C++ adapter (include/ladds_amd.hpp) the way LADDS's simulation loop drives
+// AutoPas + OrbitPropagator: build a container, an accumulator, an integrator, integrate,
+// then iterate a CollisionFunctor over the container (Simulation.cpp:122-148, 211-222, 297).
+//
+// Reads a scenario, one command per line, from the file named by argv[1] and writes results
+// to stdout; tests/test_cpp_adapter.py writes the scenarios and checks the output.
+//
+//   container <cutoff_km> <precision> <scan_variant>
+//   accumulator <KEP J2 C22 S22 SOL LUN SRP DRAG as 0/1> <t0>
+//   integrator <dt>
+//   particle <rx ry rz vx vy vz id activity mass radius_m bc_inv parent cr>
+//   integrate <steps>
+//   functor <cutoff dt cdf min_det_radius_m evasion_cutoff_km>   -> "hit|evaded idA idB d2 mx my mz"
+//   state                                                         -> "state id rx ry rz vx vy vz"
+//   burnups <min_altitude_km>                                     -> "removed n"
+//   simulate <first_iteration> <steps> <tpcd> <min_altitude_km>   -> "stats ..." and "log ..."
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <string>
+
+#include "ladds_amd.hpp"
+
+using namespace ladds_amd;
+
+static void print_pairs(const char *tag, const CollisionFunctor::CollisionCollectionT &c) {
+  for (const auto &[a, b, d2, mid] : c)
+    std::printf("%s %zu %zu %.17g %.17g %.17g %.17g\n", tag, a->getID(), b->getID(), d2, mid[0], mid[1], mid[2]);
+}
+
+int main(int argc, char **argv) {
+  if (argc != 2) {
+    std::fprintf(stderr, "usage: %s scenario.txt\n", argv[0]);
+    return 2;
+  }
+  std::ifstream in(argv[1]);
+  if (!in) {
+    std::fprintf(stderr, "cannot open %s\n", argv[1]);
+    return 2;
+  }
+  std::unique_ptr<Container> container;
+  std::unique_ptr<Acceleration::AccelerationAccumulator> accumulator;
+  std::unique_ptr<YoshidaIntegrator> integrator;
+  std::string line;
+  try {
+    while (std::getline(in, line)) {
+      std::istringstream ls(line);
+      std::string cmd;
+      if (!(ls >> cmd) || cmd[0] == '#') continue;
+      if (cmd == "container") {
+        ContainerConfig cc;
+        ls >> cc.cutoff >> cc.precision >> cc.scanVariant;
+        container = std::make_unique<Container>(cc);
+      } else if (cmd == "accumulator") {
+        std::array<bool, 8> comp{};
+        for (auto &c : comp) {
+          int b = 0;
+          ls >> b;
+          c = b != 0;
+        }
+        double t0 = 0.0;
+        ls >> t0;
+        accumulator = std::make_unique<Acceleration::AccelerationAccumulator>(comp, *container, t0, nullptr);
+      } else if (cmd == "integrator") {
+        double dt = 0.0;
+        ls >> dt;
+        integrator = std::make_unique<YoshidaIntegrator>(*container, *accumulator, dt);
+      } else if (cmd == "particle") {
+        double rx, ry, rz, vx, vy, vz, mass, radius, bc, cr;
+        unsigned long long id, parent;
+        int act;
+        ls >> rx >> ry >> rz >> vx >> vy >> vz >> id >> act >> mass >> radius >> bc >> parent >> cr;
+        container->addParticle(Particle({rx, ry, rz}, {vx, vy, vz}, id, "", static_cast<Particle::ActivityState>(act),
+                                        mass, radius, bc, parent, cr));
+      } else if (cmd == "integrate") {
+        int steps = 0;
+        ls >> steps;
+        for (int s = 0; s < steps; ++s) integrator->integrate(false);
+      } else if (cmd == "functor") {
+        double cutoff, dt, cdf, min_det, evasion;
+        ls >> cutoff >> dt >> cdf >> min_det >> evasion;
+        CollisionFunctor f(cutoff, dt, cdf, min_det, evasion);
+        container->iteratePairwise(&f);
+        print_pairs("hit", f.getCollisions());
+        print_pairs("evaded", f.getEvadedCollisions());
+      } else if (cmd == "state") {
+        for (auto it = container->cbegin(); it != container->cend(); ++it) {
+          const auto &r = it->getPosition();
+          const auto &v = it->getVelocity();
+          std::printf("state %zu %.17g %.17g %.17g %.17g %.17g %.17g\n", it->getID(), r[0], r[1], r[2], v[0], v[1],
+                      v[2]);
+        }
+      } else if (cmd == "burnups") {
+        double h = 0.0;
+        ls >> h;
+        std::printf("removed %zu\n", container->deleteBurnUps(h));
+      } else if (cmd == "simulate") {
+        unsigned long long first = 0;
+        int steps = 0, tpcd = 1;
+        double h = 0.0;
+        ls >> first >> steps >> tpcd >> h;
+        const ladds_run_stats st = container->simulate(first, steps, tpcd, h);
+        std::printf("stats %llu %llu %llu\n", (unsigned long long)st.collisions, (unsigned long long)st.evasions,
+                    (unsigned long long)st.remaining);
+        for (const ladds_logged_pair &lp : container->fetchLog())
+          std::printf("log %llu %s %llu %llu %.17g\n", (unsigned long long)lp.iteration, lp.evaded ? "evaded" : "hit",
+                      (unsigned long long)lp.pair.id_a, (unsigned long long)lp.pair.id_b, lp.pair.d2);
+      } else {
+        std::fprintf(stderr, "unknown command: %s\n", cmd.c_str());
+        return 2;
+      }
+    }
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -126,3 +126,14 @@ def test_planted_pairs():
         b = k * (nbase // p)
         d = np.linalg.norm(r[nbase + k] - r[b])
         assert 0.29 < d < 0.91
+
+
+def test_cpp_adapter_header_compiles(tmp_path):
+    """include/ladds_amd.hpp (the C++ adapter) compiles stand-alone with -Wall -Wextra."""
+    import subprocess
+    from conftest import ROOT
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "ladds_amd.hpp"\nint main() { return 0; }\n')
+    res = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                          f"-I{ROOT / 'include'}", str(src)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
