@@ -213,6 +213,38 @@ LADDS_API int ladds_synchronize(ladds_ctx *ctx);
 /* HIP stream the context launches on (hipStream_t as void*). */
 LADDS_API void *ladds_stream(ladds_ctx *ctx);
 
+/* ---- differentiable propagation (FP64): d(final state)/d(initial state, bc_inv) ----
+ * No LADDS counterpart: the reference propagates forward only (YoshidaIntegrator::integrate,
+ * Simulation.cpp:297).  The forward pass is the FP64 step of ladds_propagate; the backward
+ * pass is a hand-written adjoint of the Yoshida-4 step with checkpoints every
+ * checkpoint_every steps.  All array arguments are DEVICE pointers on the plan's device;
+ * vectors are [3][n] blocks (x row, y row, z row); stream is a hipStream_t (NULL: default).
+ * aom, bc_inv, cr, activity may be NULL (0, 0, 1, passive); g_rT / g_vT may be NULL (zero);
+ * g_bc may be NULL (not computed).  The caller owns the checkpoint buffer
+ * (ladds_adjoint_checkpoint_doubles() doubles) between forward and backward; the plan owns
+ * a workspace, so calls on one plan must not overlap. */
+typedef struct ladds_adjoint ladds_adjoint;
+typedef struct ladds_adjoint_config {
+  uint64_t n;               /* objects                                      */
+  int32_t nsteps;           /* Yoshida steps of one forward pass            */
+  int32_t checkpoint_every; /* steps between stored states (0: 64)          */
+  double dt;                /* step [s]                                     */
+  double t0;                /* start time [s since J2000]                   */
+  uint32_t components;      /* LADDS_KEP..LADDS_DRAG bits                   */
+  int32_t device;           /* HIP device ordinal                           */
+} ladds_adjoint_config;
+
+LADDS_API int ladds_adjoint_create(const ladds_adjoint_config *cfg, ladds_adjoint **out);
+LADDS_API int ladds_adjoint_destroy(ladds_adjoint *plan);
+LADDS_API const char *ladds_adjoint_last_error(const ladds_adjoint *plan);
+LADDS_API uint64_t ladds_adjoint_checkpoint_doubles(const ladds_adjoint *plan);
+LADDS_API int ladds_adjoint_forward(ladds_adjoint *plan, const double *r0, const double *v0, const double *aom,
+                                    const double *bc_inv, const double *cr, const int32_t *activity, double *rT,
+                                    double *vT, double *checkpoints, void *stream);
+LADDS_API int ladds_adjoint_backward(ladds_adjoint *plan, const double *aom, const double *bc_inv, const double *cr,
+                                     const int32_t *activity, const double *checkpoints, const double *g_rT,
+                                     const double *g_vT, double *g_r0, double *g_v0, double *g_bc, void *stream);
+
 /* ---- seeded synthetic populations (host only, no device needed) ----
  * workload 0: Keplerian elements as the 16,384-object case (perigee altitude U[200,2000] km,
  *             e U[0,0.02], inclination mixture, radius log-U[0.05,5] m, bc_inv from c_D = 2.2);

@@ -95,6 +95,18 @@ class RunStats(C.Structure):
     ]
 
 
+class AdjointConfig(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64),
+        ("nsteps", C.c_int32),
+        ("checkpoint_every", C.c_int32),
+        ("dt", C.c_double),
+        ("t0", C.c_double),
+        ("components", C.c_uint32),
+        ("device", C.c_int32),
+    ]
+
+
 assert C.sizeof(Pair) == 56 and C.sizeof(LoggedPair) == 72
 
 # numpy structured dtypes with the same layout (for zero-copy downloads)
@@ -137,6 +149,12 @@ _SIGNATURES = {
     "ladds_kernel_stats": (C.c_int, [_ctx, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "ladds_synchronize": (C.c_int, [_ctx]),
     "ladds_stream": (C.c_void_p, [_ctx]),
+    "ladds_adjoint_create": (C.c_int, [C.POINTER(AdjointConfig), C.POINTER(_ctx)]),
+    "ladds_adjoint_destroy": (C.c_int, [_ctx]),
+    "ladds_adjoint_last_error": (C.c_char_p, [_ctx]),
+    "ladds_adjoint_checkpoint_doubles": (C.c_uint64, [_ctx]),
+    "ladds_adjoint_forward": (C.c_int, [_ctx] + [C.c_void_p] * 10),
+    "ladds_adjoint_backward": (C.c_int, [_ctx] + [C.c_void_p] * 11),
     "ladds_generate_population": (C.c_int, [C.c_int, C.c_uint64, C.c_size_t, C.c_size_t] + [_dp] * 11),
 }
 

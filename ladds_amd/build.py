@@ -33,6 +33,7 @@ SOURCES = [
     "conjunction.hip",
     "compact.hip",
     "tiles.hip",
+    "adjoint.hip",
 ]
 
 COMMON = [

@@ -53,6 +53,16 @@ struct AtmTable {
   double inv_scale[kAtmRows];
 };
 
+// Host-side fill of the table from the constants (uploaded to __constant__ memory by the
+// kernels' translation units).
+inline void fill_atm_table(AtmTable &t) {
+  for (int i = 0; i < kAtmPadded; ++i) t.h0[i] = i < kAtmRows ? kAtmH0[i] : __builtin_inf();
+  for (int i = 0; i < kAtmRows; ++i) {
+    t.rho[i] = kAtmRho[i];
+    t.inv_scale[i] = 1.0 / kAtmScale[i];
+  }
+}
+
 // Row of the table: largest i with h0[i] <= h (0 below the table and for NaN).
 __device__ __forceinline__ int atm_row(double h, const AtmTable &t) {
   int i = 0;

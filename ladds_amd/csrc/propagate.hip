@@ -105,11 +105,7 @@ __global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
 
 hipError_t upload_atmosphere() {
   AtmTable t;
-  for (int i = 0; i < kAtmPadded; ++i) t.h0[i] = i < kAtmRows ? kAtmH0[i] : INFINITY;
-  for (int i = 0; i < kAtmRows; ++i) {
-    t.rho[i] = kAtmRho[i];
-    t.inv_scale[i] = 1.0 / kAtmScale[i];
-  }
+  fill_atm_table(t);
   return hipMemcpyToSymbol(HIP_SYMBOL(c_atm), &t, sizeof(t));
 }
 

@@ -5,7 +5,7 @@
  * for what each function follows and what pins it.  Build: oracle/Makefile
  * (gcc -O2 -ffp-contract=off -fno-fast-math -fopenmp).
  */
-#include "oracle.h"
+#include "oracle_internal.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -13,35 +13,6 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
-
-/* ------------------------------------------------------------------------- */
-/* constants                                                                  */
-/* ------------------------------------------------------------------------- */
-/* Earth gravity field: EGM96 (GM, reference radius, unnormalised C20/C22/S22
- * obtained from the normalised coefficients C20=-4.84165371736e-4,
- * C22=2.43914352398e-6, S22=-1.40016683654e-6).  GM = 398600.4418 km^3/s^2 is
- * also the value the reference's 1-s Kepler vector (SimulationTest.cpp:39-57)
- * reproduces. */
-#define MU_EARTH 398600.4418
-#define R_EARTH 6378.1363
-#define J2_EARTH 1.0826266835531513e-3
-#define C22_EARTH 1.574460374564035e-6
-#define S22_EARTH -9.03803806638557e-7
-/* third bodies (DE405 GM values) and the astronomical unit (IAU 2012) */
-#define GM_SUN 1.32712440018e11
-#define GM_MOON 4902.800066
-#define AU_KM 149597870.7
-/* solar radiation pressure at 1 AU [N/m^2], Montenbruck & Gill eq. 3.69 */
-#define P_SUN 4.56e-6
-/* Earth rotation rate [rad/s] and GMST at J2000.0 (280.46061837504 deg) */
-#define OMEGA_EARTH 7.2921158553e-5
-#define THETA_J2000 4.894961212823756
-/* obliquity of the ecliptic at J2000 (IAU 1976: 84381.448 arcsec) */
-#define OBLIQUITY 0.40909280422232897
-#define SEC_PER_CENTURY 3155760000.0
-#define DEG (3.14159265358979323846 / 180.0)
-#define ARCSEC (DEG / 3600.0)
-#define TWO_PI 6.28318530717958647692
 
 double or_constant(const char *name) {
   static const struct {
@@ -56,28 +27,15 @@ double or_constant(const char *name) {
   return NAN;
 }
 
-/* Vallado (2013) Table 8-4: base altitude [km], nominal density [kg/m^3], scale height [km] */
-static const double ATM_H0[28] = {0,   25,  30,  40,  50,  60,  70,  80,  90,  100, 110, 120, 130, 140,
-                                  150, 180, 200, 250, 300, 350, 400, 450, 500, 600, 700, 800, 900, 1000};
-static const double ATM_RHO[28] = {1.225,     3.899e-2,  1.774e-2,  3.972e-3,  1.057e-3,  3.206e-4,  8.770e-5,
-                                   1.905e-5,  3.396e-6,  5.297e-7,  9.661e-8,  2.438e-8,  8.484e-9,  3.845e-9,
-                                   2.070e-9,  5.464e-10, 2.789e-10, 7.248e-11, 2.418e-11, 9.518e-12, 3.725e-12,
-                                   1.585e-12, 6.967e-13, 1.454e-13, 3.614e-14, 1.170e-14, 5.245e-15, 3.019e-15};
-static const double ATM_H[28] = {7.249,  6.349,  6.682,  7.554,  8.382,  7.714,  6.549,  5.799,  5.382,  5.877,
-                                 7.263,  9.473,  12.636, 16.149, 22.523, 29.740, 37.105, 45.546, 53.628, 53.298,
-                                 58.515, 60.828, 63.822, 71.835, 88.667, 124.64, 181.05, 268.00};
+/* the force model and the object step, instantiated for double */
+#define OR_T double
+#define OR_SQRT sqrt
+#define OR_EXP exp
+#define OR_RE(x) (x)
+#define OR_FN(name) real_##name
+#include "force_model.inc"
 
-double or_density(double h) {
-  int i = 0;
-  /* largest i with h0[i] <= h; below 0 km the first row is used */
-  for (int k = 27; k >= 0; --k) {
-    if (h >= ATM_H0[k]) {
-      i = k;
-      break;
-    }
-  }
-  return ATM_RHO[i] * exp(-(h - ATM_H0[i]) / ATM_H[i]);
-}
+double or_density(double h) { return real_density(h); }
 
 /* ------------------------------------------------------------------------- */
 /* ephemerides                                                                */
@@ -135,12 +93,7 @@ double or_earth_rotation_angle(double t) {
 /* ------------------------------------------------------------------------- */
 /* force model                                                                */
 /* ------------------------------------------------------------------------- */
-typedef struct {
-  double ct, st;                /* cos/sin Earth rotation angle */
-  double sun[3], moon[3];       /* positions [km] */
-} or_env;
-
-static void env_at(double t, or_env *e) {
+void or_env_at(double t, or_env *e) {
   const double th = or_earth_rotation_angle(t);
   e->ct = cos(th);
   e->st = sin(th);
@@ -148,107 +101,12 @@ static void env_at(double t, or_env *e) {
   or_moon_position(t, e->moon);
 }
 
-static void third_body(const double r[3], const double s[3], double gm, double a[3]) {
-  /* Montenbruck & Gill eq. 3.37 */
-  const double d[3] = {s[0] - r[0], s[1] - r[1], s[2] - r[2]};
-  const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  const double dn = sqrt(d2);
-  const double s2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
-  const double sn = sqrt(s2);
-  const double fd = gm / (d2 * dn);
-  const double fs = gm / (s2 * sn);
-  a[0] += fd * d[0] - fs * s[0];
-  a[1] += fd * d[1] - fs * s[1];
-  a[2] += fd * d[2] - fs * s[2];
-}
-
-static void accel_env(const double r[3], const double v[3], unsigned mask, const or_params *p, const or_env *e,
-                      double a[3]) {
-  a[0] = a[1] = a[2] = 0.0;
-  if (p->activity == OR_EVASIVE_PRESERVING) mask &= OR_KEP; /* Particle.h:49-52 */
-  const double x = r[0], y = r[1], z = r[2];
-  const double r2 = x * x + y * y + z * z;
-  const double rn = sqrt(r2);
-  if (mask & OR_KEP) {
-    const double f = -MU_EARTH / (r2 * rn);
-    a[0] += f * x;
-    a[1] += f * y;
-    a[2] += f * z;
-  }
-  if (mask & OR_J2) {
-    /* gradient of -J2 mu R^2 / (2 r^3) (3 z^2/r^2 - 1) (M&G eq. 3.33 with n=2, m=0) */
-    const double z2r2 = z * z / r2;
-    const double f = -(1.5 * J2_EARTH * MU_EARTH * R_EARTH * R_EARTH) / (r2 * r2 * rn);
-    a[0] += f * x * (1.0 - 5.0 * z2r2);
-    a[1] += f * y * (1.0 - 5.0 * z2r2);
-    a[2] += f * z * (3.0 - 5.0 * z2r2);
-  }
-  if (mask & (OR_C22 | OR_S22)) {
-    /* degree-2 order-2 tesseral terms in the Earth-fixed frame (rotation about z by theta):
-     * U22 = 3 mu R^2 (C22 (x^2 - y^2) + 2 S22 x y) / r^5  (M&G eq. 3.8 with P22 = 3 cos^2 phi) */
-    const double xe = e->ct * x + e->st * y;
-    const double ye = -e->st * x + e->ct * y;
-    const double ze = z;
-    const double inv_r5 = 1.0 / (r2 * r2 * rn);
-    const double inv_r7 = inv_r5 / r2;
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    if (mask & OR_C22) {
-      const double k = 3.0 * MU_EARTH * R_EARTH * R_EARTH * C22_EARTH;
-      const double q = xe * xe - ye * ye;
-      ax += k * (2.0 * xe * inv_r5 - 5.0 * xe * q * inv_r7);
-      ay += k * (-2.0 * ye * inv_r5 - 5.0 * ye * q * inv_r7);
-      az += k * (-5.0 * ze * q * inv_r7);
-    }
-    if (mask & OR_S22) {
-      const double k = 6.0 * MU_EARTH * R_EARTH * R_EARTH * S22_EARTH;
-      const double q = xe * ye;
-      ax += k * (ye * inv_r5 - 5.0 * xe * q * inv_r7);
-      ay += k * (xe * inv_r5 - 5.0 * ye * q * inv_r7);
-      az += k * (-5.0 * ze * q * inv_r7);
-    }
-    a[0] += e->ct * ax - e->st * ay;
-    a[1] += e->st * ax + e->ct * ay;
-    a[2] += az;
-  }
-  if (mask & OR_SOL) third_body(r, e->sun, GM_SUN, a);
-  if (mask & OR_LUN) third_body(r, e->moon, GM_MOON, a);
-  if (mask & OR_SRP) {
-    /* cannonball SRP, M&G eq. 3.75, with the cylindrical shadow model (M&G Sec. 3.4.2) */
-    const double *s = e->sun;
-    const double d[3] = {x - s[0], y - s[1], z - s[2]};
-    const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    const double dn = sqrt(d2);
-    const double s2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
-    const double sr = s[0] * x + s[1] * y + s[2] * z;
-    int lit = 1;
-    if (sr < 0.0) {
-      const double perp2 = r2 - sr * sr / s2;
-      if (perp2 < R_EARTH * R_EARTH) lit = 0;
-    }
-    if (lit) {
-      const double f = P_SUN * 1.0e3 * AU_KM * AU_KM * p->cr * p->aom / (d2 * dn);
-      a[0] += f * d[0];
-      a[1] += f * d[1];
-      a[2] += f * d[2];
-    }
-  }
-  if (mask & OR_DRAG) {
-    /* M&G eq. 3.97 with co-rotating atmosphere: v_rel = v - omega x r */
-    const double vr[3] = {v[0] + OMEGA_EARTH * y, v[1] - OMEGA_EARTH * x, v[2]};
-    const double vn = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
-    const double rho = or_density(rn - R_EARTH);
-    const double f = -0.5e3 * p->bc_inv * rho * vn;
-    a[0] += f * vr[0];
-    a[1] += f * vr[1];
-    a[2] += f * vr[2];
-  }
-}
-
 void or_acceleration(const double r[3], const double v[3], double t, unsigned mask, const or_params *p,
                      double a[3]) {
   or_env e;
-  env_at(t, &e);
-  accel_env(r, v, mask, p, &e, a);
+  or_env_at(t, &e);
+  const real_params q = {p->aom, p->bc_inv, p->cr, p->activity};
+  real_accel_env(r, v, mask, &q, &e, a);
 }
 
 void or_yoshida_coefficients(double c[4], double d[3]) {
@@ -270,31 +128,17 @@ void or_yoshida_steps(size_t n, double *rx, double *ry, double *rz, double *vx, 
   or_env *env = (or_env *)malloc(sizeof(or_env) * 3 * (size_t)(nsteps > 0 ? nsteps : 1));
   for (int s = 0; s < nsteps; ++s) {
     const double t = t0 + (double)s * dt;
-    for (int k = 0; k < 3; ++k) env_at(t + tau[k] * dt, &env[3 * s + k]);
+    for (int k = 0; k < 3; ++k) or_env_at(t + tau[k] * dt, &env[3 * s + k]);
   }
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(static)
 #endif
   for (long long i = 0; i < (long long)n; ++i) {
-    or_params p = {aom ? aom[i] : 0.0, bc_inv ? bc_inv[i] : 0.0, cr ? cr[i] : 1.0, activity ? activity[i] : 0};
-    double r[3] = {rx[i], ry[i], rz[i]}, v[3] = {vx[i], vy[i], vz[i]}, a[3];
-    for (int s = 0; s < nsteps; ++s) {
-      for (int k = 0; k < 3; ++k) {
-        const double cdt = c[k] * dt, ddt = d[k] * dt;
-        r[0] += cdt * v[0];
-        r[1] += cdt * v[1];
-        r[2] += cdt * v[2];
-        accel_env(r, v, mask, &p, &env[3 * s + k], a);
-        v[0] += ddt * a[0];
-        v[1] += ddt * a[1];
-        v[2] += ddt * a[2];
-      }
-      const double cdt = c[3] * dt;
-      r[0] += cdt * v[0];
-      r[1] += cdt * v[1];
-      r[2] += cdt * v[2];
-    }
+    const real_params p = {aom ? aom[i] : 0.0, bc_inv ? bc_inv[i] : 0.0, cr ? cr[i] : 1.0,
+                           activity ? activity[i] : 0};
+    double r[3] = {rx[i], ry[i], rz[i]}, v[3] = {vx[i], vy[i], vz[i]};
+    real_yoshida_object(r, v, &p, env, c, d, dt, mask, nsteps);
     rx[i] = r[0];
     ry[i] = r[1];
     rz[i] = r[2];

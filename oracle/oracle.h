@@ -19,6 +19,9 @@
  *   or_yoshida_steps                     YoshidaIntegrator::integrate (OrbitPropagator @ b90f3bae, not in
  *                                        the tree; used at src/ladds/Simulation.cpp:145,297).  Restated from
  *                                        H. Yoshida, Phys. Lett. A 150 (1990) 262, 4th-order composition.
+ *   or_yoshida_vjp                       gradient of or_yoshida_steps by complex-step differentiation
+ *                                        of the same arithmetic (gradient.c): the reference for the
+ *                                        engine's adjoint, which has no LADDS counterpart
  *   or_acceleration                      AccelerationAccumulator (OrbitPropagator @ b90f3bae; flags built at
  *                                        src/ladds/Simulation.cpp:127-135).  Components restated from
  *                                        Montenbruck & Gill, Satellite Orbits (2000) Sec. 3.2-3.5 and the
@@ -84,6 +87,18 @@ void or_yoshida_coefficients(double c[4], double d[3]);
 void or_yoshida_steps(size_t n, double *rx, double *ry, double *rz, double *vx, double *vy, double *vz,
                       const double *aom, const double *bc_inv, const double *cr, const int32_t *activity,
                       double t0, double dt, unsigned mask, int nsteps, int nthreads);
+
+/* Vector-Jacobian product of or_yoshida_steps: with (r_T, v_T) the state after nsteps
+ * steps from (r, v), writes o_x = sum_k g_k d(state_T)_k / dx for x in r0, v0, bc_inv
+ * (o_bc may be NULL).  Complex-step derivatives (h = 1e-20), 7 complex propagations per
+ * object; pinned against central finite differences of or_yoshida_steps and the closed
+ * form of the straight line (all components off). */
+void or_yoshida_vjp(size_t n, const double *rx, const double *ry, const double *rz, const double *vx,
+                    const double *vy, const double *vz, const double *aom, const double *bc_inv, const double *cr,
+                    const int32_t *activity, double t0, double dt, unsigned mask, int nsteps, const double *g_rx,
+                    const double *g_ry, const double *g_rz, const double *g_vx, const double *g_vy,
+                    const double *g_vz, double *o_rx, double *o_ry, double *o_rz, double *o_vx, double *o_vy,
+                    double *o_vz, double *o_bc, int nthreads);
 
 /* ---- conjunction predicate ---- */
 void or_distance_by_linear_interpolation(const double ri[3], const double vi[3], const double rj[3],

@@ -54,6 +54,8 @@ def _load():
     lib.or_yoshida_coefficients.argtypes = [_dp, _dp]
     lib.or_yoshida_steps.argtypes = [C.c_size_t] + [_dp] * 6 + [_dp, _dp, _dp, C.POINTER(C.c_int32), C.c_double,
                                                                C.c_double, C.c_uint, C.c_int, C.c_int]
+    lib.or_yoshida_vjp.argtypes = [C.c_size_t] + [_dp] * 9 + [C.POINTER(C.c_int32), C.c_double, C.c_double,
+                                                              C.c_uint, C.c_int] + [_dp] * 13 + [C.c_int]
     lib.or_distance_by_linear_interpolation.argtypes = [_dp, _dp, _dp, _dp, C.c_double, _dp, _dp]
     lib.or_collision_functor.restype = C.c_int
     lib.or_collision_functor.argtypes = [C.POINTER(Objects), C.c_size_t, C.c_size_t, C.POINTER(FunctorCfg),
@@ -118,6 +120,23 @@ def propagate(r, v, t0, dt, mask, nsteps, aom=None, bc_inv=None, cr=None, activi
     lib.or_yoshida_steps(n, *[_p(c) for c in cols], *[None if a is None else _p(a) for a in args],
                          None if act is None else _p(act, C.c_int32), t0, dt, mask, nsteps, nthreads)
     return np.stack(cols[:3], 1), np.stack(cols[3:], 1)
+
+
+def vjp(r, v, t0, dt, mask, nsteps, g_r, g_v, aom=None, bc_inv=None, cr=None, activity=None, nthreads=0):
+    """Complex-step vector-Jacobian product of ``propagate``: returns (g_r0, g_v0, g_bc)
+    for upstream gradients g_r, g_v of the final (r, v)."""
+    r = np.asarray(r, np.float64)
+    v = np.asarray(v, np.float64)
+    n = r.shape[0]
+    cols = [np.ascontiguousarray(x[:, k], np.float64) for x in (r, v) for k in range(3)]
+    gcols = [np.ascontiguousarray(x[:, k], np.float64) for x in (np.asarray(g_r), np.asarray(g_v)) for k in range(3)]
+    args = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (aom, bc_inv, cr)]
+    act = None if activity is None else np.ascontiguousarray(activity, np.int32)
+    outs = [np.zeros(n) for _ in range(7)]
+    lib.or_yoshida_vjp(n, *[_p(c) for c in cols], *[None if a is None else _p(a) for a in args],
+                       None if act is None else _p(act, C.c_int32), t0, dt, mask, nsteps,
+                       *[_p(g) for g in gcols], *[_p(o) for o in outs], nthreads)
+    return np.stack(outs[:3], 1), np.stack(outs[3:6], 1), outs[6]
 
 
 def distance_lines(ri, vi, rj, vj, dt):
