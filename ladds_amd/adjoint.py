@@ -26,6 +26,18 @@ from ._abi import ALL_COMPONENTS, lib
 __all__ = ["AdjointPlan", "YoshidaFlow", "propagate"]
 
 
+def _hip_runtimes() -> set:
+    with open("/proc/self/maps") as f:
+        return {ln.split()[-1] for ln in f if "libamdhip64" in ln}
+
+
+# torch bundles its own HIP runtime; the engine library must be bound to the same one
+# (import torch before ladds_amd), or the two runtimes contend for the device.
+if len(_hip_runtimes()) > 1:
+    raise ImportError("two HIP runtimes are loaded (torch's and ROCm's): import torch before ladds_amd "
+                      "so that the engine library binds to torch's runtime")
+
+
 class AdjointPlan:
     """Per-(n, nsteps, dt, t0, components) state: the sub-step environments on the device
     and the backward workspace.  One plan serves any number of forward/backward pairs;

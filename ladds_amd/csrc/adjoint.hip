@@ -100,14 +100,16 @@ __device__ __forceinline__ void yoshida_step(State &s, const ObjParams &p, unsig
   s.z += cdt[3] * s.vz;
 }
 
-// Hessian-vector product of a central term gm (d/|d|^3) with d = sign (r - s):
-// d/dr [f d] . w = f (sign w - 3 tau d),  tau = (d.w) / |d|^2.
+// Hessian-vector product of a central term f d, f = k/|d|^3, with d = sign (r - s):
+// d(d/|d|^3)/dd = (I - 3 d d^T/|d|^2)/|d|^3 and dd/dr = sign I, so
+// d/dr [f d] . w = sign f (w - 3 tau d),  tau = (d.w) / |d|^2.
 __device__ __forceinline__ void central_hvp(double dx, double dy, double dz, double f, double inv_d2, double sign,
                                             double wx, double wy, double wz, double &gx, double &gy, double &gz) {
   const double tau = (dx * wx + dy * wy + dz * wz) * inv_d2;
-  gx += f * (sign * wx - 3.0 * tau * dx);
-  gy += f * (sign * wy - 3.0 * tau * dy);
-  gz += f * (sign * wz - 3.0 * tau * dz);
+  const double sf = sign * f;
+  gx += sf * (wx - 3.0 * tau * dx);
+  gy += sf * (wy - 3.0 * tau * dy);
+  gz += sf * (wz - 3.0 * tau * dz);
 }
 
 // J_r^T w, J_v^T w and (da/dbc_inv).w of acceleration<kFormFp64> at (r, v).

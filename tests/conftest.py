@@ -2,6 +2,11 @@ import json
 import sys
 from pathlib import Path
 
+# One HIP runtime per process: torch ships its own libamdhip64 (same soname as ROCm's).
+# Importing torch first makes the engine library bind to that runtime, so the torch-based
+# adjoint tests and the engine tests share one runtime in this process.
+import torch  # noqa: F401
+
 import numpy as np
 import pytest
 

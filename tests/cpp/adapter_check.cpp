@@ -30,6 +30,14 @@ static void print_pairs(const char *tag, const CollisionFunctor::CollisionCollec
     std::printf("%s %zu %zu %.17g %.17g %.17g %.17g\n", tag, a->getID(), b->getID(), d2, mid[0], mid[1], mid[2]);
 }
 
+// Every field of the line was read and nothing but blanks follows.
+static bool parsed(std::istringstream &ls, const std::string &line) {
+  std::string rest;
+  if (!ls.fail() && !(ls >> rest)) return true;
+  std::fprintf(stderr, "malformed line: %s\n", line.c_str());
+  return false;
+}
+
 int main(int argc, char **argv) {
   if (argc != 2) {
     std::fprintf(stderr, "usage: %s scenario.txt\n", argv[0]);
@@ -49,9 +57,18 @@ int main(int argc, char **argv) {
       std::istringstream ls(line);
       std::string cmd;
       if (!(ls >> cmd) || cmd[0] == '#') continue;
+      if (cmd != "container" && !container) {
+        std::fprintf(stderr, "%s before container\n", cmd.c_str());
+        return 2;
+      }
+      if ((cmd == "integrator" && !accumulator) || (cmd == "integrate" && !integrator)) {
+        std::fprintf(stderr, "%s before its accumulator/integrator\n", cmd.c_str());
+        return 2;
+      }
       if (cmd == "container") {
         ContainerConfig cc;
         ls >> cc.cutoff >> cc.precision >> cc.scanVariant;
+        if (!parsed(ls, line)) return 2;
         container = std::make_unique<Container>(cc);
       } else if (cmd == "accumulator") {
         std::array<bool, 8> comp{};
@@ -62,25 +79,30 @@ int main(int argc, char **argv) {
         }
         double t0 = 0.0;
         ls >> t0;
+        if (!parsed(ls, line)) return 2;
         accumulator = std::make_unique<Acceleration::AccelerationAccumulator>(comp, *container, t0, nullptr);
       } else if (cmd == "integrator") {
         double dt = 0.0;
         ls >> dt;
+        if (!parsed(ls, line)) return 2;
         integrator = std::make_unique<YoshidaIntegrator>(*container, *accumulator, dt);
       } else if (cmd == "particle") {
         double rx, ry, rz, vx, vy, vz, mass, radius, bc, cr;
         unsigned long long id, parent;
         int act;
         ls >> rx >> ry >> rz >> vx >> vy >> vz >> id >> act >> mass >> radius >> bc >> parent >> cr;
+        if (!parsed(ls, line)) return 2;
         container->addParticle(Particle({rx, ry, rz}, {vx, vy, vz}, id, "", static_cast<Particle::ActivityState>(act),
                                         mass, radius, bc, parent, cr));
       } else if (cmd == "integrate") {
         int steps = 0;
         ls >> steps;
+        if (!parsed(ls, line)) return 2;
         for (int s = 0; s < steps; ++s) integrator->integrate(false);
       } else if (cmd == "functor") {
         double cutoff, dt, cdf, min_det, evasion;
         ls >> cutoff >> dt >> cdf >> min_det >> evasion;
+        if (!parsed(ls, line)) return 2;
         CollisionFunctor f(cutoff, dt, cdf, min_det, evasion);
         container->iteratePairwise(&f);
         print_pairs("hit", f.getCollisions());
@@ -95,12 +117,14 @@ int main(int argc, char **argv) {
       } else if (cmd == "burnups") {
         double h = 0.0;
         ls >> h;
+        if (!parsed(ls, line)) return 2;
         std::printf("removed %zu\n", container->deleteBurnUps(h));
       } else if (cmd == "simulate") {
         unsigned long long first = 0;
         int steps = 0, tpcd = 1;
         double h = 0.0;
         ls >> first >> steps >> tpcd >> h;
+        if (!parsed(ls, line)) return 2;
         const ladds_run_stats st = container->simulate(first, steps, tpcd, h);
         std::printf("stats %llu %llu %llu\n", (unsigned long long)st.collisions, (unsigned long long)st.evasions,
                     (unsigned long long)st.remaining);

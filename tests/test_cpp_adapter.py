@@ -45,8 +45,9 @@ def _run(tmp_path, lines):
 
 
 def _particle(r, v, pid, act=0, mass=1.0, radius=1.0, bc=0.0, parent=UMAX, cr=1.0):
-    vals = [*map(repr, map(float, r)), *map(repr, map(float, v))]
-    return f"particle {' '.join(vals)} {pid} {act} {mass!r} {radius!r} {bc!r} {parent} {cr!r}"
+    vals = [repr(float(x)) for x in (*r, *v)]
+    mass, radius, bc, cr = (repr(float(x)) for x in (mass, radius, bc, cr))
+    return f"particle {' '.join(vals)} {int(pid)} {int(act)} {mass} {radius} {bc} {int(parent)} {cr}"
 
 
 def _header(cutoff, dt, comps=(1, 0, 0, 0, 0, 0, 0, 0), precision=la.FP64, variant=0, t0=0.0):

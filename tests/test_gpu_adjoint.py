@@ -88,7 +88,7 @@ def test_straight_line_closed_form():
     gr, gv, gbc, rT, _ = _gpu_vjp(pop, 0, 100, 10.0, g_r, g_v, every=7)
     np.testing.assert_allclose(gr, g_r, rtol=1e-13, atol=1e-13)
     np.testing.assert_allclose(gv, g_v + 1000.0 * g_r, rtol=1e-12, atol=1e-12)
-    np.testing.assert_allclose(rT, pop.r() + 1000.0 * pop.v(), rtol=1e-13)
+    np.testing.assert_allclose(rT, pop.r() + 1000.0 * pop.v(), rtol=1e-12)
 
 
 @pytest.mark.parametrize("every", [1, 7, 64, 1440])
