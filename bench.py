@@ -62,8 +62,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--objects", type=int, default=1 << 20)
     p.add_argument("--profile-steps", type=int, default=500)
-    p.add_argument("--cpu-steps", type=int, default=12)
+    p.add_argument("--cpu-steps", type=int, default=60, help="steps of the CPU baseline sample (~15 s on 16 threads)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-propagation leg")
     p.add_argument("--forms", default="fp64,split")
     return p.parse_args()
 
@@ -125,12 +126,70 @@ def cpu_baseline(pop, steps):
                       f"workload (propagate + linked-cell scan), {el:.1f} s on {threads} host threads"}
 
 
+def adjoint_leg(la, reps=3):
+    """Differentiable propagation: 4,096 objects, full force model in FP64, dt = 60 s, 1,440
+    steps (one day), checkpoints every 64 steps; loss = sum |r_T - target|^2 with targets =
+    nominal final state + N(0, 100 m).  Reports gradient-steps/s (objects x steps per
+    forward + backward pass, device time from events on torch's stream) and the relative
+    error of the gradient against the oracle's FP64 complex-step gradient."""
+    import torch
+    from ladds_amd import adjoint
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as ol
+
+    n, steps, dt = 4096, 1440, 60.0
+    pop = la.generate_population(1, 0x1ADD5 + 4, n)
+    dev = torch.device("cuda:0")
+    t = lambda x: torch.tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
+    plan = adjoint.AdjointPlan(n, steps, dt, components=la.ALL_COMPONENTS, checkpoint_every=64)
+    with torch.no_grad():
+        rT0, _ = adjoint.propagate(t(pop.r()), t(pop.v()), t(pop.bc_inv), plan, aom=t(pop.aom), cr=t(pop.cr))
+    target = rT0 + 0.1 * torch.randn(rT0.shape, dtype=torch.float64, device=dev,
+                                     generator=torch.Generator(device=dev).manual_seed(4))
+    times = []
+    aom_t, cr_t = t(pop.aom), t(pop.cr)
+    for rep in range(reps + 1):
+        r0, v0, bc = t(pop.r()).requires_grad_(), t(pop.v()).requires_grad_(), t(pop.bc_inv).requires_grad_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rT, _ = adjoint.propagate(r0, v0, bc, plan, aom=aom_t, cr=cr_t)
+        loss = ((rT - target) ** 2).sum()
+        loss.backward()
+        e1.record()
+        torch.cuda.synchronize()
+        if rep:
+            times.append(e0.elapsed_time(e1) * 1e-3)
+    sec = float(np.median(times))
+    g_r = 2.0 * (rT.detach().cpu().numpy() - target.cpu().numpy())
+    t0 = time.perf_counter()
+    threads = min(16, len(os.sched_getaffinity(0)))
+    cr_, cv_, cbc = ol.vjp(pop.r(), pop.v(), 0.0, dt, la.ALL_COMPONENTS, steps, g_r, np.zeros_like(g_r),
+                           aom=pop.aom, bc_inv=pop.bc_inv, cr=pop.cr, nthreads=threads)
+    cpu_sec = time.perf_counter() - t0
+    got = np.concatenate([r0.grad.cpu().numpy(), v0.grad.cpu().numpy()], 1)
+    ref = np.concatenate([cr_, cv_], 1)
+    rel = np.max(np.abs(got - ref), 1) / np.max(np.abs(ref), 1)
+    bc_rel = np.abs(bc.grad.cpu().numpy() - cbc) / np.max(np.abs(cbc))
+    plan.close()
+    return {"workload": f"{n} objects, full force model FP64, dt {dt:g} s, {steps} steps, checkpoints every 64",
+            "gradient_steps_per_s": n * steps / sec, "ms_per_gradient": sec * 1e3,
+            "grad_rel_err_max": float(rel.max()), "grad_rel_err_median": float(np.median(rel)),
+            "bc_grad_rel_err_max": float(bc_rel.max()),
+            "cpu_reference": {"kind": "port", "method": "complex-step FP64 (7 complex propagations per object)",
+                              "gradient_steps_per_s": n * steps / cpu_sec, "cores": threads}}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the engine has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
+    if not a.no_adjoint:
+        # torch ships its own HIP runtime: importing it first binds the engine library to it
+        print("# importing torch", file=sys.stderr, flush=True)
+        import torch  # noqa: F401
     import ladds_amd as la
 
     if la.device_count() < 1:
@@ -190,6 +249,9 @@ def main():
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                            "avg_us": kern[dom]["avg_us"], "bytes_per_object": per_obj}
     out["kernels_us"] = {form: {k: round(v["avg_us"], 2) for k, v in res[form]["kernels"].items()} for form in res}
+    if not a.no_adjoint:
+        out["adjoint"] = adjoint_leg(la)
+        print(f"# adjoint: {json.dumps(out['adjoint'])}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pop, a.cpu_steps)
     print(json.dumps(out))
