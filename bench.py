@@ -66,7 +66,21 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-propagation leg")
     p.add_argument("--forms", default="fp64,split")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                   help="first copy MIB MiB device-to-device 3 times (a plain copy kernel of known size, "
+                        "for scaling hardware byte counters), then run the bench")
     return p.parse_args()
+
+
+def calibrate_copy(mib):
+    import torch
+
+    x = torch.ones(mib << 20, dtype=torch.uint8, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"# copied {mib} MiB x 3 (device to device)", file=sys.stderr, flush=True)
 
 
 def make_engine(la, precision):
@@ -186,10 +200,12 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the engine has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
-    if not a.no_adjoint:
+    if not a.no_adjoint or a.calibrate_copy:
         # torch ships its own HIP runtime: importing it first binds the engine library to it
         print("# importing torch", file=sys.stderr, flush=True)
         import torch  # noqa: F401
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
     import ladds_amd as la
 
     if la.device_count() < 1:

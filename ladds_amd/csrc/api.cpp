@@ -271,6 +271,10 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   tp.plane_bits = g.by + g.bz;
   tp.nplanes = 1u << g.bx;
   tp.lds_header_words = pair_tiles_header_words(tp);
+  // ladds_config.block_size overrides; by default FP64 records (32 B) leave room for one
+  // workgroup per CU, which then needs 16 waves, while the split form fits several.
+  const int bs = ctx->cfg.block_size;
+  tp.pair_threads = (bs == 256 || bs == 512 || bs == 1024) ? (uint32_t)bs : (ctx->form == kFormFp64 ? 1024u : 256u);
   const size_t rec = (ctx->form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
   size_t pair_max = 0;
   for (size_t p = 0; p < plane_count.size(); ++p)

@@ -51,6 +51,7 @@ struct TileParams {
   uint32_t nplanes;          // 2^bx
   uint32_t cap_items;        // records of one plane pair that fit the pair kernel's LDS
   uint32_t lds_header_words; // LDS words before the record array in the pair kernel
+  uint32_t pair_threads;     // pair-kernel workgroup size: 256, 512 or 1024
 };
 
 // Sorted record of the plain FP64 form and of the split form: the position the

@@ -32,8 +32,11 @@ namespace {
 
 constexpr int kPartThreads = 1024;
 constexpr int kPartItems = kPartitionTile / kPartThreads;  // objects per thread in the partition
-constexpr int kPairThreads = 256;
-constexpr int kPairRegItems = 8;  // records per thread kept in registers between the two sort passes
+// Pair-kernel workgroups of 256, 512 or 1024 threads (TileParams::pair_threads).  The LDS
+// footprint of a plane pair fixes how many workgroups share a CU; larger workgroups give
+// the latency-bound scan more waves when only one fits (the FP64 records are twice the size).
+constexpr int kMaxPairThreads = 1024;
+constexpr int kPairRegCapacity = 2048;  // records kept in registers between the two sort passes
 
 // In-place exclusive scan of a[0..n) in LDS by the whole block; returns the total.
 // `wsum` holds blockDim/64 + 1 entries.
@@ -181,66 +184,71 @@ __device__ __forceinline__ bool within(const RecOf<FORM> &a, const RecOf<FORM> &
   }
 }
 
-// Scan one record range [lo, hi) of a sorted plane array.
-template <int FORM, bool DIGEST>
-__device__ __forceinline__ void scan_range(const PairTileArgs &a, const RecOf<FORM> &me, const RecOf<FORM> *arr,
-                                           uint32_t lo, uint32_t hi, ScanAcc &acc) {
-#pragma unroll 1
-  for (uint32_t q = lo; q < hi; ++q) {
-    const RecOf<FORM> other = arr[q];
-    if (within<FORM>(me, other, a.sink.fp)) handle_candidate<DIGEST>(a.sink, me.idx, other.idx, acc);
-  }
-}
-
 // Bucket ranges of the sorted plane arrays: cur[b] = end of bucket b after placement.
 __device__ __forceinline__ uint32_t bucket_begin(const uint32_t *cur, uint32_t b) { return b ? cur[b - 1] : 0u; }
-
-// Scan the up to three contiguous bucket runs covering z in {fz-1, fz, fz+1} (mod Kz) of one
-// row of a sorted plane array; records before `from` are skipped.
-template <int FORM, bool DIGEST>
-__device__ __forceinline__ void scan_row(const PairTileArgs &a, const RecOf<FORM> &me, const RecOf<FORM> *arr,
-                                         const uint32_t *cur, uint32_t row, uint32_t fz, uint32_t mz, uint32_t from,
-                                         ScanAcc &acc) {
-  if (fz != 0u && fz != mz) {
-    scan_range<FORM, DIGEST>(a, me, arr, max(bucket_begin(cur, row | (fz - 1u)), from), cur[row | (fz + 1u)], acc);
-  } else if (fz == 0u) {
-    scan_range<FORM, DIGEST>(a, me, arr, max(bucket_begin(cur, row), from), cur[row | 1u], acc);
-    scan_range<FORM, DIGEST>(a, me, arr, max(bucket_begin(cur, row | mz), from), cur[row | mz], acc);
-  } else {  // fz == mz
-    scan_range<FORM, DIGEST>(a, me, arr, max(bucket_begin(cur, row | (mz - 1u)), from), cur[row | mz], acc);
-    scan_range<FORM, DIGEST>(a, me, arr, max(bucket_begin(cur, row), from), cur[row], acc);
-  }
-}
 
 // Cells have side >= the search radius R, so partners within R lie in the 3x3x3 cells
 // around an object.  Within its own plane a pair is taken from the lower sorted
 // position; pairs with the next plane are all taken here (the next plane's own scan
 // never looks back), pairs with the previous plane by the previous plane's workgroup.
+// Per object: six rows (three y-rows of its own plane, three of the next); in each row
+// the buckets fz-1, fz, fz+1 (mod Kz) are one contiguous run of the sorted array, or two
+// runs where the fold wraps.  The rows are walked by one loop so that the candidate path
+// (the exact predicate and the wavefront compaction) exists once in the code.
 template <int FORM, bool DIGEST>
 __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FORM> *own, const RecOf<FORM> *halo,
                                            const uint32_t *cur_own, const uint32_t *cur_halo, uint32_t n_own,
-                                           ScanAcc &acc) {
+                                           uint32_t threads, ScanAcc &acc) {
   const GridParams &g = a.g;
   const uint32_t my = (1u << g.by) - 1u, mz = (1u << g.bz) - 1u;
   const uint32_t sy = g.bz;
 #pragma unroll 1
-  for (uint32_t s = threadIdx.x; s < n_own; s += blockDim.x) {
+  for (uint32_t s = threadIdx.x; s < n_own; s += threads) {
     const RecOf<FORM> me = own[s];
     double x, y, z;
     binning_point_of(me, x, y, z);
     const uint32_t lb = cell_of(x, y, z, g).bucket;
     const uint32_t fy = (lb >> sy) & my, fz = lb & mz;
 #pragma unroll 1
-    for (uint32_t dy = 0; dy < 3; ++dy) {
-      const uint32_t row = ((fy + dy - 1u) & my) << sy;
-      scan_row<FORM, DIGEST>(a, me, own, cur_own, row, fz, mz, s + 1u, acc);
-      scan_row<FORM, DIGEST>(a, me, halo, cur_halo, row, fz, mz, 0u, acc);
+    for (uint32_t kind = 0; kind < 6u; ++kind) {
+      const bool next_plane = (kind & 1u) != 0u;
+      const RecOf<FORM> *arr = next_plane ? halo : own;
+      const uint32_t *cur = next_plane ? cur_halo : cur_own;
+      const uint32_t from = next_plane ? 0u : s + 1u;
+      const uint32_t row = ((fy + (kind >> 1) - 1u) & my) << sy;
+      uint32_t lo0, hi0, lo1 = 0u, hi1 = 0u;
+      if (fz != 0u && fz != mz) {
+        lo0 = bucket_begin(cur, row | (fz - 1u));
+        hi0 = cur[row | (fz + 1u)];
+      } else if (fz == 0u) {
+        lo0 = bucket_begin(cur, row);
+        hi0 = cur[row | 1u];
+        lo1 = bucket_begin(cur, row | mz);
+        hi1 = cur[row | mz];
+      } else {
+        lo0 = bucket_begin(cur, row | (mz - 1u));
+        hi0 = cur[row | mz];
+        lo1 = bucket_begin(cur, row);
+        hi1 = cur[row];
+      }
+      uint32_t q = max(lo0, from), end = hi0;
+#pragma unroll 1
+      for (int part = 0; part < 2; ++part) {
+#pragma unroll 1
+        for (; q < end; ++q) {
+          const RecOf<FORM> other = arr[q];
+          if (within<FORM>(me, other, a.sink.fp)) handle_candidate<DIGEST>(a.sink, me.idx, other.idx, acc);
+        }
+        q = max(lo1, from);
+        end = hi1;
+      }
     }
   }
 }
 
-template <int FORM, bool DIGEST>
-__global__ __launch_bounds__(kPairThreads) void k_pair_tiles(PairTileArgs a) {
+template <int FORM, bool DIGEST, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
+  constexpr int kPairRegItems = kPairRegCapacity / THREADS;
   using REC = RecOf<FORM>;
   extern __shared__ uint32_t lds[];
   const TileParams &tp = a.tp;
@@ -253,8 +261,8 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_tiles(PairTileArgs a) {
   uint32_t *src_next = src_own + T;
   uint32_t *cur_own = src_next + T;        // L bucket cursors
   uint32_t *cur_next = cur_own + L;
-  uint32_t *wsum = cur_next + L;           // kPairThreads / 64 + 1
-  uint32_t *misc = wsum + (kPairThreads / 64 + 1);  // [0] scratch offset
+  uint32_t *wsum = cur_next + L;           // THREADS / 64 + 1
+  uint32_t *misc = wsum + (kMaxPairThreads / 64 + 1);  // [0] scratch offset
   REC *lds_items = reinterpret_cast<REC *>(lds + tp.lds_header_words);
 
   const REC *items = static_cast<const REC *>(a.items);
@@ -307,18 +315,18 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_tiles(PairTileArgs a) {
       binning_point_of(r, x, y, z);
       return cell_of(x, y, z, a.g).bucket & mloc;
     };
-    if (total <= (uint32_t)(kPairRegItems * kPairThreads)) {
+    if (total <= (uint32_t)kPairRegCapacity) {
       // records stay in registers between the histogram and the placement pass
       REC reg[kPairRegItems];
       uint32_t lb[kPairRegItems];
 #pragma unroll
       for (int k = 0; k < kPairRegItems; ++k) {
-        const uint32_t j = threadIdx.x + k * kPairThreads;
+        const uint32_t j = threadIdx.x + k * THREADS;
         if (j < total) reg[k] = items[loc(j)];
       }
 #pragma unroll
       for (int k = 0; k < kPairRegItems; ++k) {
-        const uint32_t j = threadIdx.x + k * kPairThreads;
+        const uint32_t j = threadIdx.x + k * THREADS;
         if (j < total) {
           lb[k] = local_bucket(reg[k]);
           atomicAdd(&(j < n_own ? cur_own : cur_next)[lb[k]], 1u);
@@ -329,7 +337,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_tiles(PairTileArgs a) {
       block_exclusive_scan(cur_next, L, wsum);
 #pragma unroll
       for (int k = 0; k < kPairRegItems; ++k) {
-        const uint32_t j = threadIdx.x + k * kPairThreads;
+        const uint32_t j = threadIdx.x + k * THREADS;
         if (j < total) {
           const bool is_own = j < n_own;
           const uint32_t pos = atomicAdd(&(is_own ? cur_own : cur_next)[lb[k]], 1u);
@@ -363,9 +371,9 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_tiles(PairTileArgs a) {
     }
     // cur[b] now holds the end of bucket b
     if (in_lds)
-      scan_plane<FORM, DIGEST>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, acc);
+      scan_plane<FORM, DIGEST>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, THREADS, acc);
     else
-      scan_plane<FORM, DIGEST>(a, own, halo, cur_own, cur_next, n_own, acc);
+      scan_plane<FORM, DIGEST>(a, own, halo, cur_own, cur_next, n_own, THREADS, acc);
   }
   flush_counters<DIGEST>(a.sink, acc);
 }
@@ -396,25 +404,47 @@ hipError_t launch_tile_partition(int form, const DeviceObjects &o, const uint32_
   return hipGetLastError();
 }
 
-hipError_t configure_tile_kernels() {
-  const int max_lds = 160 * 1024;
+template <int THREADS>
+hipError_t set_pair_lds_limit(int max_lds) {
   hipError_t e = hipSuccess;
   auto set = [&](const void *f) {
     if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
   };
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, true>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, true>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, true, THREADS>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, true, THREADS>));
+  return e;
+}
+
+hipError_t configure_tile_kernels() {
+  const int max_lds = 160 * 1024;
+  hipError_t e = set_pair_lds_limit<256>(max_lds);
+  if (e == hipSuccess) e = set_pair_lds_limit<512>(max_lds);
+  if (e == hipSuccess) e = set_pair_lds_limit<1024>(max_lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_partition<RecordF64>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_partition<RecordF32>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
   return e;
 }
 
 uint32_t pair_tiles_header_words(const TileParams &tp) {
   const uint32_t L = 1u << tp.plane_bits;
-  uint32_t w = 4 * tp.ntiles + 2 + 2 * L + kPairThreads / 64 + 1 + 4;
+  uint32_t w = 4 * tp.ntiles + 2 + 2 * L + kMaxPairThreads / 64 + 1 + 4;
   return (w + 7u) & ~7u;  // 32-byte alignment of the record array
+}
+
+template <int THREADS>
+void launch_pair_tiles_as(int form, bool digest, const PairTileArgs &a, size_t shm, hipStream_t s) {
+  const dim3 grid(a.tp.nplanes), block(THREADS);
+  const bool f64 = (form == kFormFp64);
+  if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false, THREADS>), grid, block, shm, s, a);
+  if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true, THREADS>), grid, block, shm, s, a);
+  if (!f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, false, THREADS>), grid, block, shm, s, a);
+  if (!f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, true, THREADS>), grid, block, shm, s, a);
 }
 
 hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, const GridParams &g,
@@ -433,12 +463,19 @@ hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, cons
   a.arena_top = arena_top;
   const size_t rec = (form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
   const size_t shm = sizeof(uint32_t) * tp.lds_header_words + rec * tp.cap_items;
-  const dim3 grid(tp.nplanes), block(kPairThreads);
-  const bool f64 = (form == kFormFp64);
-  if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false>), grid, block, shm, s, a);
-  if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true>), grid, block, shm, s, a);
-  if (!f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, false>), grid, block, shm, s, a);
-  if (!f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, true>), grid, block, shm, s, a);
+  switch (tp.pair_threads) {
+    case 256:
+      launch_pair_tiles_as<256>(form, digest, a, shm, s);
+      break;
+    case 512:
+      launch_pair_tiles_as<512>(form, digest, a, shm, s);
+      break;
+    case 1024:
+      launch_pair_tiles_as<1024>(form, digest, a, shm, s);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
