@@ -203,6 +203,25 @@ LADDS_API int ladds_clear_log(ladds_ctx *ctx);
  * compacting the arrays on the device (order preserved). */
 LADDS_API int ladds_delete_burnups(ladds_ctx *ctx, double min_altitude_km, size_t *removed);
 
+/* ---- autotuning ----
+ * The counterpart of AutoPas's tuning phase (autopas/tuning, driven by the
+ * iteratePairwise calls of Simulation.cpp:211-222, whose return value reports
+ * "still tuning"): time the plane-tile conjunction scan on the current state for
+ * candidate cell sizes (1, 1.25, 1.5, 2 x the search radius), plane shapes (log2 buckets
+ * per x-plane around the population-derived default), partition tile sizes (4096, 8192,
+ * 16384 objects) and pair-kernel workgroup sizes, and keep the fastest for this context.  Pair results do not depend on the choice. */
+typedef struct ladds_tuning {
+  double cell_size_km;   /* chosen cell side [km]                               */
+  int32_t plane_bits;    /* chosen log2 buckets per x-plane                     */
+  int32_t pair_threads;  /* chosen pair-kernel workgroup size                   */
+  double scan_us;        /* device time of one detection with the choice [us]   */
+  double default_us;     /* the same for the untuned default [us]               */
+  int32_t tried;         /* configurations timed                                */
+  int32_t tile_size;     /* chosen objects per partition tile                   */
+} ladds_tuning;
+
+LADDS_API int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuning *out);
+
 /* ---- measurement ---- */
 /* Record HIP events around every kernel launch; ladds_kernel_stats reports per kernel kind. */
 LADDS_API int ladds_set_profiling(ladds_ctx *ctx, int enable);

@@ -286,6 +286,15 @@ class Engine:
         self._check(lib.ladds_delete_burnups(self._h, float(min_altitude_km), C.byref(removed)), "delete_burnups")
         return removed.value
 
+    def autotune(self, dt_interp: float | None = None, repetitions: int = 3) -> dict:
+        """Pick the plane-tile cell size, plane shape and pair workgroup size by timing the
+        conjunction scan on the current state (``ladds_autotune``)."""
+        t = _abi.Tuning()
+        dt = self.config.dt if dt_interp is None else dt_interp
+        self._check(lib.ladds_autotune(self._h, float(dt), int(repetitions), C.byref(t)), "autotune")
+        return {"cell_size_km": t.cell_size_km, "plane_bits": t.plane_bits, "pair_threads": t.pair_threads,
+                "tile_size": t.tile_size, "scan_us": t.scan_us, "default_us": t.default_us, "tried": t.tried}
+
     # -- measurement ----------------------------------------------------
     def set_profiling(self, enable: bool):
         self._check(lib.ladds_set_profiling(self._h, int(bool(enable))), "set_profiling")

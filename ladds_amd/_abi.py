@@ -95,6 +95,18 @@ class RunStats(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("cell_size_km", C.c_double),
+        ("plane_bits", C.c_int32),
+        ("pair_threads", C.c_int32),
+        ("scan_us", C.c_double),
+        ("default_us", C.c_double),
+        ("tried", C.c_int32),
+        ("tile_size", C.c_int32),
+    ]
+
+
 class AdjointConfig(C.Structure):
     _fields_ = [
         ("n", C.c_uint64),
@@ -145,6 +157,7 @@ _SIGNATURES = {
     "ladds_fetch_log": (C.c_int, [_ctx, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "ladds_clear_log": (C.c_int, [_ctx]),
     "ladds_delete_burnups": (C.c_int, [_ctx, C.c_double, C.POINTER(C.c_size_t)]),
+    "ladds_autotune": (C.c_int, [_ctx, C.c_double, C.c_int, C.POINTER(Tuning)]),
     "ladds_set_profiling": (C.c_int, [_ctx, C.c_int]),
     "ladds_kernel_stats": (C.c_int, [_ctx, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "ladds_synchronize": (C.c_int, [_ctx]),

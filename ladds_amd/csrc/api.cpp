@@ -93,6 +93,12 @@ struct ladds_ctx {
   unsigned long long *log_count = nullptr;
   ScanCounters *counters = nullptr;
 
+  // Autotuned plane-tile shape (ladds_autotune); zero / negative values keep the defaults.
+  double tune_cell_km = 0.0;
+  int tune_pb = -1;
+  int tune_threads = 0;
+  int tune_tile = 0;
+
   uint64_t step_count = 0;  // steps since t0
   double c[4], dcoef[3];
 
@@ -212,6 +218,7 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   const double radius = search_radius(ctx->form, ctx->cfg.cutoff_km);
   const double min_cell = (tiles ? 1.0 : 2.0) * radius * (1.0 + 1e-9);
   double cell = ctx->cfg.cell_size_km > 0.0 ? ctx->cfg.cell_size_km : min_cell;
+  if (tiles && ctx->tune_cell_km > 0.0) cell = ctx->tune_cell_km;
   if (!(cell >= min_cell))
     return ctx->fail(LADDS_E_INVALID, tiles ? "cell_size_km must be >= the search radius (cutoff, or the split "
                                               "prefilter radius)"
@@ -229,7 +236,8 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
     const double nx = std::min(1.0e6, 2.0 * xmax / cell + 3.0);
     g.bx = std::min<uint32_t>(14u, std::max<uint32_t>(2u, ceil_log2((uint64_t)nx)));
     const uint64_t used = std::min<uint64_t>((uint64_t)nx, 1ull << g.bx);
-    const uint32_t pb = std::min<uint32_t>(12u, std::max<uint32_t>(4u, ceil_log2(std::max<size_t>(n, 1) / used + 1)));
+    uint32_t pb = std::min<uint32_t>(12u, std::max<uint32_t>(4u, ceil_log2(std::max<size_t>(n, 1) / used + 1)));
+    if (ctx->tune_pb >= 4 && ctx->tune_pb <= 12) pb = (uint32_t)ctx->tune_pb;
     g.by = pb / 2;
     g.bz = pb - g.by;
     bits = g.bx + pb;
@@ -266,8 +274,8 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
     ctx->tile_sum_alloc = ts_need;
   }
   TileParams tp{};
-  tp.tile_size = kPartitionTile;
-  tp.ntiles = (uint32_t)std::max<size_t>(1, (n + kPartitionTile - 1) / kPartitionTile);
+  tp.tile_size = (ctx->tune_tile == 4096 || ctx->tune_tile == 16384) ? (uint32_t)ctx->tune_tile : kPartitionTile;
+  tp.ntiles = (uint32_t)std::max<size_t>(1, (n + tp.tile_size - 1) / tp.tile_size);
   tp.plane_bits = g.by + g.bz;
   tp.nplanes = 1u << g.bx;
   tp.lds_header_words = pair_tiles_header_words(tp);
@@ -275,6 +283,8 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   // workgroup per CU, which then needs 16 waves, while the split form fits several.
   const int bs = ctx->cfg.block_size;
   tp.pair_threads = (bs == 256 || bs == 512 || bs == 1024) ? (uint32_t)bs : (ctx->form == kFormFp64 ? 1024u : 256u);
+  if (ctx->tune_threads == 256 || ctx->tune_threads == 512 || ctx->tune_threads == 1024)
+    tp.pair_threads = (uint32_t)ctx->tune_threads;
   const size_t rec = (ctx->form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
   size_t pair_max = 0;
   for (size_t p = 0; p < plane_count.size(); ++p)
@@ -705,6 +715,100 @@ static int run_detection_sync(ladds_ctx *ctx, double dt_interp, bool digest, Sca
   HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->log_count, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   *entries = cnt;
+  return LADDS_OK;
+}
+
+// Device time of one detection (keys, partition, pair scan in digest mode) on the
+// current state: the minimum over `reps` runs after one warm-up run.
+static int time_detection(ladds_ctx *ctx, double dt_interp, int reps, double *us) {
+  hipEvent_t e0, e1;
+  HIPCHK(ctx, hipEventCreate(&e0));
+  HIPCHK(ctx, hipEventCreate(&e1));
+  double best = 1e30;
+  int rc = LADDS_OK;
+  for (int r = 0; r <= reps && rc == LADDS_OK; ++r) {
+    (void)hipEventRecord(e0, ctx->stream);
+    rc = launch_detection(ctx, false, dt_interp, true, 0);
+    (void)hipEventRecord(e1, ctx->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = ctx->fail(LADDS_E_HIP, "detection timing failed");
+    float ms = 0.f;
+    if (rc == LADDS_OK && r > 0 && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) best = std::min(best, 1000.0 * ms);
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *us = best;
+  return rc;
+}
+
+int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuning *out) {
+  if (!ctx || repetitions < 1) return LADDS_E_INVALID;
+  if (ctx->n < 2) return ctx->fail(LADDS_E_STATE, "set objects before autotuning");
+  if (ctx->variant != kVariantPlaneTiles) return ctx->fail(LADDS_E_STATE, "autotuning covers the plane-tile scan");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t n = ctx->n;
+  std::vector<double> rx(n);
+  HIPCHK(ctx, hipMemcpy(rx.data(), ctx->d.rx, sizeof(double) * n, hipMemcpyDeviceToHost));
+  // the untuned default first
+  ctx->tune_cell_km = 0.0;
+  ctx->tune_pb = -1;
+  ctx->tune_threads = 0;
+  ctx->tune_tile = 0;
+  int rc = setup_grid(ctx, n, rx.data());
+  if (rc != LADDS_OK) return rc;
+  const double base_cell = 1.0 / ctx->grid.inv_cell;
+  const int base_pb = (int)(ctx->grid.by + ctx->grid.bz);
+  double default_us = 0.0;
+  rc = time_detection(ctx, dt_interp, repetitions, &default_us);
+  if (rc != LADDS_OK) return rc;
+  double best_us = default_us, best_cell = 0.0;
+  int best_pb = -1, best_threads = 0, tried = 1;
+  const double cell_factors[] = {1.0, 1.25, 1.5, 2.0};
+  const int threads[] = {256, 512, 1024};
+  const int tiles[] = {4096, 8192, 16384};
+  int best_tile = 0;
+  for (double f : cell_factors) {
+    for (int dpb = -1; dpb <= 2; ++dpb) {
+      const int pb = base_pb + dpb;
+      if (pb < 4 || pb > 12) continue;
+      for (int t : threads) for (int tile : tiles) {
+        ctx->tune_cell_km = base_cell * f;
+        ctx->tune_pb = pb;
+        ctx->tune_threads = t;
+        ctx->tune_tile = tile;
+        rc = setup_grid(ctx, n, rx.data());
+        if (rc != LADDS_OK) return rc;
+        double us = 0.0;
+        rc = time_detection(ctx, dt_interp, repetitions, &us);
+        if (rc != LADDS_OK) return rc;
+        ++tried;
+        if (us < best_us) {
+          best_us = us;
+          best_cell = ctx->tune_cell_km;
+          best_pb = pb;
+          best_threads = t;
+          best_tile = tile;
+        }
+      }
+    }
+  }
+  ctx->tune_cell_km = best_cell;
+  ctx->tune_pb = best_pb;
+  ctx->tune_threads = best_threads;
+  ctx->tune_tile = best_tile;
+  rc = setup_grid(ctx, n, rx.data());
+  if (rc != LADDS_OK) return rc;
+  HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (out) {
+    out->cell_size_km = 1.0 / ctx->grid.inv_cell;
+    out->plane_bits = (int32_t)(ctx->grid.by + ctx->grid.bz);
+    out->pair_threads = (int32_t)ctx->tiles.pair_threads;
+    out->tile_size = (int32_t)ctx->tiles.tile_size;
+    out->scan_us = best_us;
+    out->default_us = default_us;
+    out->tried = tried;
+  }
   return LADDS_OK;
 }
 

@@ -31,7 +31,6 @@ namespace ladds {
 namespace {
 
 constexpr int kPartThreads = 1024;
-constexpr int kPartItems = kPartitionTile / kPartThreads;  // objects per thread in the partition
 // Pair-kernel workgroups of 256, 512 or 1024 threads (TileParams::pair_threads).  The LDS
 // footprint of a plane pair fixes how many workgroups share a CU; larger workgroups give
 // the latency-bound scan more waves when only one fits (the FP64 records are twice the size).
@@ -100,7 +99,7 @@ __global__ __launch_bounds__(256) void k_bin_keys(const double *rx, const double
   keys[i] = cell_of(x, y, z, g).bucket;
 }
 
-template <typename REC>
+template <typename REC, int kPartItems>
 __global__ __launch_bounds__(kPartThreads) void k_tile_partition(const double *rx, const double *ry,
                                                                   const double *rz, size_t n, const uint32_t *keys,
                                                                   TileParams tp, REC *items, uint32_t *tps,
@@ -391,17 +390,36 @@ hipError_t launch_bin_keys(int form, const DeviceObjects &o, const GridParams &g
   return hipGetLastError();
 }
 
+template <typename REC>
+hipError_t launch_tile_partition_as(const DeviceObjects &o, const uint32_t *keys, const TileParams &tp, void *items,
+                                    uint32_t *tps, uint32_t *arena_top, hipStream_t s) {
+  const size_t shm = sizeof(uint32_t) * (tp.nplanes + kPartThreads / 64 + 1);
+  const dim3 grid(tp.ntiles), block(kPartThreads);
+  REC *out = static_cast<REC *>(items);
+  switch (tp.tile_size) {
+    case 4096:
+      hipLaunchKernelGGL((k_tile_partition<REC, 4>), grid, block, shm, s, o.rx, o.ry, o.rz, o.n, keys, tp, out, tps,
+                         arena_top);
+      break;
+    case 8192:
+      hipLaunchKernelGGL((k_tile_partition<REC, 8>), grid, block, shm, s, o.rx, o.ry, o.rz, o.n, keys, tp, out, tps,
+                         arena_top);
+      break;
+    case 16384:
+      hipLaunchKernelGGL((k_tile_partition<REC, 16>), grid, block, shm, s, o.rx, o.ry, o.rz, o.n, keys, tp, out, tps,
+                         arena_top);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_tile_partition(int form, const DeviceObjects &o, const uint32_t *keys, const TileParams &tp,
                                  void *items, uint32_t *tps, uint32_t *arena_top, hipStream_t s) {
   if (o.n == 0) return hipSuccess;
-  const size_t shm = sizeof(uint32_t) * (tp.nplanes + kPartThreads / 64 + 1);
-  if (form == kFormFp64)
-    hipLaunchKernelGGL(k_tile_partition<RecordF64>, dim3(tp.ntiles), dim3(kPartThreads), shm, s, o.rx, o.ry, o.rz,
-                       o.n, keys, tp, static_cast<RecordF64 *>(items), tps, arena_top);
-  else
-    hipLaunchKernelGGL(k_tile_partition<RecordF32>, dim3(tp.ntiles), dim3(kPartThreads), shm, s, o.rx, o.ry, o.rz,
-                       o.n, keys, tp, static_cast<RecordF32 *>(items), tps, arena_top);
-  return hipGetLastError();
+  if (form == kFormFp64) return launch_tile_partition_as<RecordF64>(o, keys, tp, items, tps, arena_top, s);
+  return launch_tile_partition_as<RecordF32>(o, keys, tp, items, tps, arena_top, s);
 }
 
 template <int THREADS>
@@ -422,12 +440,15 @@ hipError_t configure_tile_kernels() {
   hipError_t e = set_pair_lds_limit<256>(max_lds);
   if (e == hipSuccess) e = set_pair_lds_limit<512>(max_lds);
   if (e == hipSuccess) e = set_pair_lds_limit<1024>(max_lds);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_partition<RecordF64>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_partition<RecordF32>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+  auto set = [&](const void *f) {
+    if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+  };
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 4>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 8>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 16>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 4>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 8>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 16>));
   return e;
 }
 

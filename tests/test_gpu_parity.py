@@ -388,3 +388,23 @@ def test_variants_agree_on_leo_population():
     assert len(ref) >= 100
     for k, (keys, _) in got.items():
         assert keys == ref, k
+
+
+@pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
+def test_autotune_keeps_the_pair_set(precision):
+    """ladds_autotune changes cell size, plane shape and workgroup size; the pair set stays
+    bit-identical to the oracle's."""
+    pop = la.generate_population(1, 0x1ADD5 + 12, 65536, planted_pairs=256)
+    cfg = dict(dt=10.0, components=la.KEP | la.J2, precision=precision, cutoff_km=10.0,
+               threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0)
+    with engine(**cfg) as eng:
+        eng.set_objects(pop)
+        eng.propagate(3)
+        t = eng.autotune(repetitions=1)
+        assert t["tried"] > 10 and t["scan_us"] <= t["default_us"]
+        hits, ev = eng.conjunctions()
+        r, v = eng.state()
+    oh, oe = ol.pairs(_oracle_objects(r, v, pop), ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0))
+    assert oh.shape[0] > 100
+    _pair_equal(ol.sort_pairs(hits), oh)
+    _pair_equal(ol.sort_pairs(ev), oe)
