@@ -131,8 +131,8 @@ typedef struct ladds_pair {
 typedef struct ladds_logged_pair {
   ladds_pair pair;
   uint64_t iteration;
-  uint32_t evaded; /* 0 collision, 1 evaded conjunction */
-  uint32_t reserved;
+  uint32_t evaded;   /* 0 collision, 1 evaded conjunction      */
+  uint32_t scenario; /* ensemble scenario (0 without scenarios) */
 } ladds_logged_pair; /* 72 bytes */
 
 typedef struct ladds_run_stats {
@@ -168,6 +168,17 @@ LADDS_API int ladds_configure(ladds_ctx *ctx, const ladds_config *cfg);
 /* AutoPas::addParticle for a whole population (SatelliteLoader.cpp:17-84). Replaces all objects. */
 LADDS_API int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *objs);
 LADDS_API size_t ladds_num_objects(const ladds_ctx *ctx);
+
+/* Batched ensembles (no LADDS counterpart; LADDS runs one scenario per process): the
+ * objects last set are the concatenation of independent scenarios, scenario s occupying
+ * indices [offsets[s], offsets[s+1]) (nscenarios + 1 entries, offsets[0] = 0,
+ * offsets[nscenarios] = ladds_num_objects()).  Each scenario gets its own cell grid;
+ * objects of different scenarios never form pairs; logged pairs carry their scenario.
+ * Burn-up compaction keeps the scenarios (ladds_get_scenarios gives the new offsets).
+ * ladds_set_objects resets to a single scenario.  Needs the plane-tile scan. */
+LADDS_API int ladds_set_scenarios(ladds_ctx *ctx, size_t nscenarios, const uint64_t *offsets);
+LADDS_API size_t ladds_num_scenarios(const ladds_ctx *ctx);
+LADDS_API int ladds_get_scenarios(const ladds_ctx *ctx, uint64_t *offsets);
 /* Read back state (caller-owned arrays of ladds_num_objects() entries; NULL skips). */
 LADDS_API int ladds_get_state(ladds_ctx *ctx, double *rx, double *ry, double *rz, double *vx, double *vy, double *vz);
 /* Ids of the current objects, in index order (after compaction). */

@@ -29,8 +29,9 @@ from ._abi import (ALL_COMPONENTS, C22, DRAG, EVASIVE, EVASIVE_PRESERVING, FP64,
                    THRESHOLD_RADIUS_SCALED, LOGGED_DTYPE, PAIR_DTYPE)
 
 __all__ = [
-    "Engine", "EngineConfig", "LaddsError", "generate_population", "device_count", "components_mask",
-    "KEP", "J2", "C22", "S22", "SOL", "LUN", "SRP", "DRAG", "ALL_COMPONENTS", "FP64", "SPLIT", "SPLIT_ALL32",
+    "Engine", "EngineConfig", "LaddsError", "generate_population", "generate_ensemble", "device_count",
+    "components_mask", "KEP", "J2", "C22", "S22", "SOL", "LUN", "SRP", "DRAG", "ALL_COMPONENTS", "FP64", "SPLIT",
+    "SPLIT_ALL32",
     "THRESHOLD_RADIUS_SCALED", "THRESHOLD_ABSOLUTE", "PASSIVE", "EVASIVE", "EVASIVE_PRESERVING",
     "PAIRS_LIST", "PAIRS_DIGEST", "PAIR_DTYPE", "LOGGED_DTYPE",
 ]
@@ -137,6 +138,19 @@ def generate_population(workload: int, seed: int, n: int, planted_pairs: int = 0
     pop.id = np.arange(n, dtype=np.uint64)
     pop.parent = np.full(n, np.iinfo(np.uint64).max, np.uint64)
     return pop
+
+
+def generate_ensemble(sizes, seed: int, workload: int = 1, planted_pairs: int = 0):
+    """Concatenated seeded scenarios (scenario s: ``generate_population(workload, seed + s, sizes[s],
+    planted_pairs)``, i.e. sizes[s] + 2 planted_pairs objects) and their offsets.  Ids are
+    ``(s << 32) | local index``, so pairs name their scenario."""
+    parts = [generate_population(workload, seed + s, int(k), planted_pairs) for s, k in enumerate(sizes)]
+    sizes = [p.n for p in parts]
+    cat = {f: np.concatenate([getattr(p, f) for p in parts]) for f in Population.__dataclass_fields__}
+    cat["id"] = np.concatenate([(np.uint64(s) << np.uint64(32)) | np.arange(k, dtype=np.uint64)
+                                for s, k in enumerate(sizes)])
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    return Population(**cat), offsets
 
 
 class Engine:
@@ -285,6 +299,18 @@ class Engine:
         removed = C.c_size_t(0)
         self._check(lib.ladds_delete_burnups(self._h, float(min_altitude_km), C.byref(removed)), "delete_burnups")
         return removed.value
+
+    def set_scenarios(self, offsets):
+        """Declare the current objects an ensemble of independent scenarios (``ladds_set_scenarios``);
+        ``offsets`` has one entry per scenario plus the total."""
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self._check(lib.ladds_set_scenarios(self._h, off.shape[0] - 1, _ptr(off, C.c_uint64)), "set_scenarios")
+
+    def scenarios(self) -> np.ndarray:
+        """Current scenario offsets (after burn-up compaction)."""
+        off = np.zeros(int(lib.ladds_num_scenarios(self._h)) + 1, np.uint64)
+        self._check(lib.ladds_get_scenarios(self._h, _ptr(off, C.c_uint64)), "get_scenarios")
+        return off
 
     def autotune(self, dt_interp: float | None = None, repetitions: int = 3) -> dict:
         """Pick the plane-tile cell size, plane shape and pair workgroup size by timing the

@@ -79,7 +79,7 @@ class LoggedPair(C.Structure):
         ("pair", Pair),
         ("iteration", C.c_uint64),
         ("evaded", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("scenario", C.c_uint32),
     ]
 
 
@@ -127,7 +127,7 @@ try:
 
     PAIR_DTYPE = np.dtype([("id_a", "<u8"), ("id_b", "<u8"), ("idx_a", "<u4"), ("idx_b", "<u4"),
                            ("d2", "<f8"), ("mid", "<f8", (3,))])
-    LOGGED_DTYPE = np.dtype([("pair", PAIR_DTYPE), ("iteration", "<u8"), ("evaded", "<u4"), ("reserved", "<u4")])
+    LOGGED_DTYPE = np.dtype([("pair", PAIR_DTYPE), ("iteration", "<u8"), ("evaded", "<u4"), ("scenario", "<u4")])
     assert PAIR_DTYPE.itemsize == 56 and LOGGED_DTYPE.itemsize == 72
 except ImportError:  # pragma: no cover
     np = None
@@ -144,6 +144,9 @@ _SIGNATURES = {
     "ladds_configure": (C.c_int, [_ctx, C.POINTER(Config)]),
     "ladds_set_objects": (C.c_int, [_ctx, C.POINTER(Objects)]),
     "ladds_num_objects": (C.c_size_t, [_ctx]),
+    "ladds_set_scenarios": (C.c_int, [_ctx, C.c_size_t, C.POINTER(C.c_uint64)]),
+    "ladds_num_scenarios": (C.c_size_t, [_ctx]),
+    "ladds_get_scenarios": (C.c_int, [_ctx, C.POINTER(C.c_uint64)]),
     "ladds_get_state": (C.c_int, [_ctx, _dp, _dp, _dp, _dp, _dp, _dp]),
     "ladds_get_ids": (C.c_int, [_ctx, C.POINTER(C.c_uint64)]),
     "ladds_get_time": (C.c_double, [_ctx]),

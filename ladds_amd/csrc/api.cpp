@@ -93,6 +93,12 @@ struct ladds_ctx {
   unsigned long long *log_count = nullptr;
   ScanCounters *counters = nullptr;
 
+  // Ensemble scenarios (ladds_set_scenarios): offsets of S scenarios, S + 1 entries; empty
+  // for a single population.  Device tile tables of the plane-tile pipeline.
+  std::vector<uint64_t> scen_off;
+  uint32_t *d_tile_begin = nullptr, *d_scen_tile = nullptr;
+  size_t tile_begin_alloc = 0, scen_tile_alloc = 0;
+
   // Autotuned plane-tile shape (ladds_autotune); zero / negative values keep the defaults.
   double tune_cell_km = 0.0;
   int tune_pb = -1;
@@ -235,19 +241,27 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
     }
     const double nx = std::min(1.0e6, 2.0 * xmax / cell + 3.0);
     g.bx = std::min<uint32_t>(14u, std::max<uint32_t>(2u, ceil_log2((uint64_t)nx)));
+    // a scenario of an ensemble is a population of its own: its planes hold ~64 objects
+    const size_t nscen = ctx->scen_off.size() > 2 ? ctx->scen_off.size() - 1 : 1;
+    const size_t per_pop = std::max<size_t>(n / nscen, 1);
+    if (nscen > 1) g.bx = std::min<uint32_t>(g.bx, std::max<uint32_t>(2u, ceil_log2(per_pop / 64 + 1)));
     const uint64_t used = std::min<uint64_t>((uint64_t)nx, 1ull << g.bx);
-    uint32_t pb = std::min<uint32_t>(12u, std::max<uint32_t>(4u, ceil_log2(std::max<size_t>(n, 1) / used + 1)));
+    uint32_t pb = std::min<uint32_t>(12u, std::max<uint32_t>(4u, ceil_log2(per_pop / used + 1)));
     if (ctx->tune_pb >= 4 && ctx->tune_pb <= 12) pb = (uint32_t)ctx->tune_pb;
     g.by = pb / 2;
     g.bz = pb - g.by;
     bits = g.bx + pb;
     if (rx && n) {
-      plane_count.assign(1u << g.bx, 0u);
-      const uint32_t mx = (1u << g.bx) - 1u;
-      for (size_t i = 0; i < n; ++i) {
-        double u = rx[i] * g.inv_cell;
-        u = std::fmin(std::fmax(u, -1.0e9), 1.0e9);
-        plane_count[(uint32_t)(int32_t)std::floor(u) & mx] += 1;
+      // per scenario plane populations (one population: a single scenario)
+      const uint32_t P = 1u << g.bx, mx = P - 1u;
+      plane_count.assign((size_t)P * nscen, 0u);
+      for (size_t sc = 0; sc < nscen; ++sc) {
+        const size_t lo = nscen > 1 ? ctx->scen_off[sc] : 0, hi = nscen > 1 ? ctx->scen_off[sc + 1] : n;
+        for (size_t i = lo; i < hi; ++i) {
+          double u = rx[i] * g.inv_cell;
+          u = std::fmin(std::fmax(u, -1.0e9), 1.0e9);
+          plane_count[sc * P + ((uint32_t)(int32_t)std::floor(u) & mx)] += 1;
+        }
       }
     }
   } else {
@@ -276,19 +290,56 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   TileParams tp{};
   tp.tile_size = (ctx->tune_tile == 4096 || ctx->tune_tile == 16384) ? (uint32_t)ctx->tune_tile : kPartitionTile;
   tp.ntiles = (uint32_t)std::max<size_t>(1, (n + tp.tile_size - 1) / tp.tile_size);
+  tp.nscen = 1;
+  if (tiles && ctx->scen_off.size() > 2) {
+    // tiles never straddle scenarios: scenario s owns tiles [scen_tile[s], scen_tile[s+1])
+    const size_t S = ctx->scen_off.size() - 1;
+    std::vector<uint32_t> tb, st(S + 1);
+    uint32_t most = 1;
+    for (size_t sc = 0; sc < S; ++sc) {
+      st[sc] = (uint32_t)tb.size();
+      for (uint64_t o = ctx->scen_off[sc]; o < ctx->scen_off[sc + 1]; o += tp.tile_size) tb.push_back((uint32_t)o);
+      most = std::max<uint32_t>(most, (uint32_t)tb.size() - st[sc]);
+    }
+    st[S] = (uint32_t)tb.size();
+    tb.push_back((uint32_t)n);
+    if (tb.size() > ctx->tile_begin_alloc) {
+      hipFree(ctx->d_tile_begin);
+      HIPCHK(ctx, dalloc(&ctx->d_tile_begin, tb.size()));
+      ctx->tile_begin_alloc = tb.size();
+    }
+    if (st.size() > ctx->scen_tile_alloc) {
+      hipFree(ctx->d_scen_tile);
+      HIPCHK(ctx, dalloc(&ctx->d_scen_tile, st.size()));
+      ctx->scen_tile_alloc = st.size();
+    }
+    HIPCHK(ctx, hipMemcpy(ctx->d_tile_begin, tb.data(), sizeof(uint32_t) * tb.size(), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->d_scen_tile, st.data(), sizeof(uint32_t) * st.size(), hipMemcpyHostToDevice));
+    tp.ntiles = (uint32_t)std::max<size_t>(1, tb.size() - 1);
+    tp.nscen = (uint32_t)S;
+    tp.max_scen_tiles = most;
+    tp.tile_begin = ctx->d_tile_begin;
+    tp.scen_tile = ctx->d_scen_tile;
+  }
   tp.plane_bits = g.by + g.bz;
   tp.nplanes = 1u << g.bx;
   tp.lds_header_words = pair_tiles_header_words(tp);
   // ladds_config.block_size overrides; by default FP64 records (32 B) leave room for one
   // workgroup per CU, which then needs 16 waves, while the split form fits several.
   const int bs = ctx->cfg.block_size;
-  tp.pair_threads = (bs == 256 || bs == 512 || bs == 1024) ? (uint32_t)bs : (ctx->form == kFormFp64 ? 1024u : 256u);
-  if (ctx->tune_threads == 256 || ctx->tune_threads == 512 || ctx->tune_threads == 1024)
+  tp.pair_threads = (bs == 128 || bs == 256 || bs == 512 || bs == 1024)
+                        ? (uint32_t)bs
+                        : (tp.nscen > 1 ? 128u : (ctx->form == kFormFp64 ? 1024u : 256u));
+  if (ctx->tune_threads == 128 || ctx->tune_threads == 256 || ctx->tune_threads == 512 || ctx->tune_threads == 1024)
     tp.pair_threads = (uint32_t)ctx->tune_threads;
   const size_t rec = (ctx->form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
   size_t pair_max = 0;
-  for (size_t p = 0; p < plane_count.size(); ++p)
-    pair_max = std::max<size_t>(pair_max, plane_count[p] + plane_count[(p + 1) % plane_count.size()]);
+  if (!plane_count.empty()) {
+    const size_t P = tp.nplanes;
+    for (size_t sc = 0; sc < plane_count.size() / P; ++sc)
+      for (size_t p = 0; p < P; ++p)
+        pair_max = std::max<size_t>(pair_max, plane_count[sc * P + p] + plane_count[sc * P + (p + 1) % P]);
+  }
   const size_t want = std::max<size_t>(1024, ((pair_max + pair_max / 4) + 255) / 256 * 256);
   const size_t fit = (kPairTileLdsBytes - 4 * (size_t)tp.lds_header_words) / rec;
   tp.cap_items = (uint32_t)std::min(want, fit);
@@ -433,10 +484,28 @@ int compact_if_needed(ladds_ctx *ctx) {
   HIPCHK(ctx, hipMemcpyAsync(&alive, ctx->n_alive, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (alive != ctx->n) {
+    std::vector<int32_t> act;
+    if (ctx->scen_off.size() > 2) {  // the flags of the pre-compaction arrays give the new offsets
+      act.resize(ctx->n);
+      HIPCHK(ctx, hipMemcpy(act.data(), ctx->d.activity, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost));
+    }
     std::swap(ctx->d, ctx->d_alt);
     ctx->d.n = alive;
     ctx->d_alt.n = 0;
     ctx->n = alive;
+    if (!act.empty()) {
+      std::vector<uint64_t> off(ctx->scen_off.size(), 0);
+      for (size_t sc = 0; sc + 1 < ctx->scen_off.size(); ++sc) {
+        uint64_t kept = 0;
+        for (uint64_t i = ctx->scen_off[sc]; i < ctx->scen_off[sc + 1]; ++i) kept += act[i] >= 0 ? 1 : 0;
+        off[sc + 1] = off[sc] + kept;
+      }
+      ctx->scen_off = off;
+      std::vector<double> rx(alive);
+      if (alive) HIPCHK(ctx, hipMemcpy(rx.data(), ctx->d.rx, sizeof(double) * alive, hipMemcpyDeviceToHost));
+      const int rc = setup_grid(ctx, alive, rx.data());
+      if (rc != LADDS_OK) return rc;
+    }
   }
   return LADDS_OK;
 }
@@ -547,6 +616,8 @@ int ladds_destroy(ladds_ctx *ctx) {
     hipEventDestroy(ev.b);
   }
   if (ctx->stream) hipStreamDestroy(ctx->stream);
+  hipFree(ctx->d_tile_begin);
+  hipFree(ctx->d_scen_tile);
   delete ctx;
   return LADDS_OK;
 }
@@ -599,6 +670,7 @@ int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
     ctx->cap = n;
   }
   {
+    ctx->scen_off.clear();  // a new population is a single scenario until ladds_set_scenarios
     const int rc = setup_grid(ctx, n, o->rx);
     if (rc != LADDS_OK) return rc;
   }
@@ -644,6 +716,41 @@ int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
   HIPCHK(ctx, up(ctx->d.id, id.data(), 8 * n));
   HIPCHK(ctx, up(ctx->d.parent, parent.data(), 8 * n));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return LADDS_OK;
+}
+
+int ladds_set_scenarios(ladds_ctx *ctx, size_t nscenarios, const uint64_t *offsets) {
+  if (!ctx || nscenarios == 0 || !offsets) return LADDS_E_INVALID;
+  if (ctx->n == 0) return ctx->fail(LADDS_E_STATE, "set objects before scenarios");
+  if (ctx->variant != kVariantPlaneTiles) return ctx->fail(LADDS_E_STATE, "scenarios need the plane-tile scan");
+  if (offsets[0] != 0 || offsets[nscenarios] != ctx->n)
+    return ctx->fail(LADDS_E_INVALID, "scenario offsets must start at 0 and end at the object count");
+  for (size_t k = 0; k < nscenarios; ++k)
+    if (offsets[k + 1] < offsets[k]) return ctx->fail(LADDS_E_INVALID, "scenario offsets must not decrease");
+  if (nscenarios > (1u << 20)) return ctx->fail(LADDS_E_INVALID, "too many scenarios");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (nscenarios == 1)
+    ctx->scen_off.clear();
+  else
+    ctx->scen_off.assign(offsets, offsets + nscenarios + 1);
+  std::vector<double> rx(ctx->n);
+  HIPCHK(ctx, hipMemcpy(rx.data(), ctx->d.rx, sizeof(double) * ctx->n, hipMemcpyDeviceToHost));
+  return setup_grid(ctx, ctx->n, rx.data());
+}
+
+size_t ladds_num_scenarios(const ladds_ctx *ctx) {
+  if (!ctx) return 0;
+  return ctx->scen_off.size() > 2 ? ctx->scen_off.size() - 1 : 1;
+}
+
+int ladds_get_scenarios(const ladds_ctx *ctx, uint64_t *offsets) {
+  if (!ctx || !offsets) return LADDS_E_INVALID;
+  if (ctx->scen_off.size() > 2) {
+    std::copy(ctx->scen_off.begin(), ctx->scen_off.end(), offsets);
+  } else {
+    offsets[0] = 0;
+    offsets[1] = ctx->n;
+  }
   return LADDS_OK;
 }
 
@@ -763,7 +870,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   double best_us = default_us, best_cell = 0.0;
   int best_pb = -1, best_threads = 0, tried = 1;
   const double cell_factors[] = {1.0, 1.25, 1.5, 2.0};
-  const int threads[] = {256, 512, 1024};
+  const int threads[] = {128, 256, 512, 1024};
   const int tiles[] = {4096, 8192, 16384};
   int best_tile = 0;
   for (double f : cell_factors) {

@@ -51,7 +51,13 @@ struct TileParams {
   uint32_t nplanes;          // 2^bx
   uint32_t cap_items;        // records of one plane pair that fit the pair kernel's LDS
   uint32_t lds_header_words; // LDS words before the record array in the pair kernel
-  uint32_t pair_threads;     // pair-kernel workgroup size: 256, 512 or 1024
+  uint32_t pair_threads;     // pair-kernel workgroup size: 128, 256, 512 or 1024
+  // Batched ensembles: independent scenarios, each with its own tiles and planes; objects
+  // of different scenarios never pair.  nscen == 1: one population, tiles of tile_size.
+  uint32_t nscen;
+  uint32_t max_scen_tiles;     // most tiles of one scenario (sizes the pair kernel's LDS)
+  const uint32_t *tile_begin;  // nscen > 1: first object of every tile (ntiles + 1 entries)
+  const uint32_t *scen_tile;   // nscen > 1: first tile of every scenario (nscen + 1 entries)
 };
 
 // Sorted record of the plain FP64 form and of the split form: the position the
@@ -88,7 +94,7 @@ struct LoggedPair {  // same layout as ladds_logged_pair
   PairRecord pair;
   uint64_t iteration;
   uint32_t evaded;
-  uint32_t reserved;
+  uint32_t scenario;
 };
 
 // Counters written by the pair scan (device memory).

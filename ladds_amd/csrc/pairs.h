@@ -119,7 +119,8 @@ struct ScanAcc {
 };
 
 template <bool DIGEST>
-__device__ __forceinline__ void handle_candidate(const PairSink &a, uint32_t oi, uint32_t oj, ScanAcc &acc) {
+__device__ __forceinline__ void handle_candidate(const PairSink &a, uint32_t oi, uint32_t oj, ScanAcc &acc,
+                                                 uint32_t scenario = 0u) {
   const uint32_t i = oi < oj ? oi : oj;
   const uint32_t j = oi < oj ? oj : oi;
   PairRecord r;
@@ -161,7 +162,7 @@ __device__ __forceinline__ void handle_candidate(const PairSink &a, uint32_t oi,
           lp.pair = r;
           lp.iteration = a.iteration;
           lp.evaded = (kind == 2) ? 1u : 0u;
-          lp.reserved = 0;
+          lp.scenario = scenario;
           a.log[pos] = lp;
         }
       }

@@ -35,7 +35,7 @@ constexpr int kPartThreads = 1024;
 // footprint of a plane pair fixes how many workgroups share a CU; larger workgroups give
 // the latency-bound scan more waves when only one fits (the FP64 records are twice the size).
 constexpr int kMaxPairThreads = 1024;
-constexpr int kPairRegCapacity = 2048;  // records kept in registers between the two sort passes
+constexpr int kPairRegCapacity = 2048;  // records kept in registers between the two sort passes (at most)
 
 // In-place exclusive scan of a[0..n) in LDS by the whole block; returns the total.
 // `wsum` holds blockDim/64 + 1 entries.
@@ -108,7 +108,15 @@ __global__ __launch_bounds__(kPartThreads) void k_tile_partition(const double *r
   uint32_t *cnt = lds;                // nplanes
   uint32_t *wsum = lds + tp.nplanes;  // kPartThreads / 64 + 1
   const uint32_t P = tp.nplanes, pb = tp.plane_bits;
-  const size_t base = (size_t)blockIdx.x * tp.tile_size;
+  // objects [base, end) of this tile
+  size_t base, end;
+  if (tp.tile_begin) {
+    base = tp.tile_begin[blockIdx.x];
+    end = tp.tile_begin[blockIdx.x + 1];
+  } else {
+    base = (size_t)blockIdx.x * tp.tile_size;
+    end = min(n, base + tp.tile_size);
+  }
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) cnt[p] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) *arena_top = 0;
   __syncthreads();
@@ -117,7 +125,7 @@ __global__ __launch_bounds__(kPartThreads) void k_tile_partition(const double *r
 #pragma unroll
   for (int k = 0; k < kPartItems; ++k) {
     const size_t i = base + threadIdx.x + (size_t)k * kPartThreads;
-    key[k] = (i < n) ? keys[i] : ~0u;
+    key[k] = (i < end) ? keys[i] : ~0u;
   }
 #pragma unroll
   for (int k = 0; k < kPartItems; ++k)
@@ -197,7 +205,7 @@ __device__ __forceinline__ uint32_t bucket_begin(const uint32_t *cur, uint32_t b
 template <int FORM, bool DIGEST>
 __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FORM> *own, const RecOf<FORM> *halo,
                                            const uint32_t *cur_own, const uint32_t *cur_halo, uint32_t n_own,
-                                           uint32_t threads, ScanAcc &acc) {
+                                           uint32_t threads, uint32_t scen, ScanAcc &acc) {
   const GridParams &g = a.g;
   const uint32_t my = (1u << g.by) - 1u, mz = (1u << g.bz) - 1u;
   const uint32_t sy = g.bz;
@@ -236,7 +244,7 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
 #pragma unroll 1
         for (; q < end; ++q) {
           const RecOf<FORM> other = arr[q];
-          if (within<FORM>(me, other, a.sink.fp)) handle_candidate<DIGEST>(a.sink, me.idx, other.idx, acc);
+          if (within<FORM>(me, other, a.sink.fp)) handle_candidate<DIGEST>(a.sink, me.idx, other.idx, acc, scen);
         }
         q = max(lo1, from);
         end = hi1;
@@ -247,12 +255,16 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
 
 template <int FORM, bool DIGEST, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
-  constexpr int kPairRegItems = kPairRegCapacity / THREADS;
+  constexpr int kPairRegItems = kPairRegCapacity / THREADS < 8 ? kPairRegCapacity / THREADS : 8;
   using REC = RecOf<FORM>;
   extern __shared__ uint32_t lds[];
   const TileParams &tp = a.tp;
-  const uint32_t T = tp.ntiles, P = tp.nplanes, L = 1u << tp.plane_bits;
-  const uint32_t plane = blockIdx.x, next = (plane + 1u) & (P - 1u);  // P >= 4
+  const uint32_t P = tp.nplanes, L = 1u << tp.plane_bits;
+  // workgroup = (scenario, plane); a scenario's tiles are [t0, t0 + T)
+  const uint32_t scen = blockIdx.x / P;
+  const uint32_t plane = blockIdx.x - scen * P, next = (plane + 1u) & (P - 1u);  // P >= 4
+  const uint32_t t0 = tp.scen_tile ? tp.scen_tile[scen] : 0u;
+  const uint32_t T = tp.scen_tile ? tp.scen_tile[scen + 1] - t0 : tp.ntiles;
   // LDS layout
   uint32_t *seg_own = lds;               // T + 1 prefix of segment lengths
   uint32_t *seg_next = seg_own + (T + 1);
@@ -266,13 +278,15 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
 
   const REC *items = static_cast<const REC *>(a.items);
   for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
-    const uint32_t *row = a.tps + (size_t)t * (P + 1);
+    const uint32_t g = t0 + t;
+    const uint32_t *row = a.tps + (size_t)g * (P + 1);
     const uint32_t o0 = row[plane], o1 = row[plane + 1];
     const uint32_t n0 = row[next], n1 = row[next + 1];
+    const uint32_t tile_base = tp.tile_begin ? tp.tile_begin[g] : g * tp.tile_size;
     seg_own[t] = o1 - o0;
     seg_next[t] = n1 - n0;
-    src_own[t] = t * tp.tile_size + o0;
-    src_next[t] = t * tp.tile_size + n0;
+    src_own[t] = tile_base + o0;
+    src_next[t] = tile_base + n0;
   }
   for (uint32_t b = threadIdx.x; b < L; b += blockDim.x) {
     cur_own[b] = 0;
@@ -314,7 +328,7 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
       binning_point_of(r, x, y, z);
       return cell_of(x, y, z, a.g).bucket & mloc;
     };
-    if (total <= (uint32_t)kPairRegCapacity) {
+    if (total <= (uint32_t)(kPairRegItems * THREADS)) {
       // records stay in registers between the histogram and the placement pass
       REC reg[kPairRegItems];
       uint32_t lb[kPairRegItems];
@@ -370,9 +384,9 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
     }
     // cur[b] now holds the end of bucket b
     if (in_lds)
-      scan_plane<FORM, DIGEST>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, THREADS, acc);
+      scan_plane<FORM, DIGEST>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, THREADS, scen, acc);
     else
-      scan_plane<FORM, DIGEST>(a, own, halo, cur_own, cur_next, n_own, THREADS, acc);
+      scan_plane<FORM, DIGEST>(a, own, halo, cur_own, cur_next, n_own, THREADS, scen, acc);
   }
   flush_counters<DIGEST>(a.sink, acc);
 }
@@ -437,7 +451,8 @@ hipError_t set_pair_lds_limit(int max_lds) {
 
 hipError_t configure_tile_kernels() {
   const int max_lds = 160 * 1024;
-  hipError_t e = set_pair_lds_limit<256>(max_lds);
+  hipError_t e = set_pair_lds_limit<128>(max_lds);
+  if (e == hipSuccess) e = set_pair_lds_limit<256>(max_lds);
   if (e == hipSuccess) e = set_pair_lds_limit<512>(max_lds);
   if (e == hipSuccess) e = set_pair_lds_limit<1024>(max_lds);
   auto set = [&](const void *f) {
@@ -454,13 +469,14 @@ hipError_t configure_tile_kernels() {
 
 uint32_t pair_tiles_header_words(const TileParams &tp) {
   const uint32_t L = 1u << tp.plane_bits;
-  uint32_t w = 4 * tp.ntiles + 2 + 2 * L + kMaxPairThreads / 64 + 1 + 4;
+  const uint32_t T = tp.nscen > 1 ? tp.max_scen_tiles : tp.ntiles;
+  uint32_t w = 4 * T + 2 + 2 * L + kMaxPairThreads / 64 + 1 + 4;
   return (w + 7u) & ~7u;  // 32-byte alignment of the record array
 }
 
 template <int THREADS>
 void launch_pair_tiles_as(int form, bool digest, const PairTileArgs &a, size_t shm, hipStream_t s) {
-  const dim3 grid(a.tp.nplanes), block(THREADS);
+  const dim3 grid(a.tp.nplanes * a.tp.nscen), block(THREADS);
   const bool f64 = (form == kFormFp64);
   if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false, THREADS>), grid, block, shm, s, a);
   if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true, THREADS>), grid, block, shm, s, a);
@@ -485,6 +501,9 @@ hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, cons
   const size_t rec = (form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
   const size_t shm = sizeof(uint32_t) * tp.lds_header_words + rec * tp.cap_items;
   switch (tp.pair_threads) {
+    case 128:
+      launch_pair_tiles_as<128>(form, digest, a, shm, s);
+      break;
     case 256:
       launch_pair_tiles_as<256>(form, digest, a, shm, s);
       break;
