@@ -103,6 +103,8 @@ typedef struct ladds_config {
   int32_t device;                    /* HIP device ordinal                                    */
   int32_t pair_output;               /* ladds_simulate: LADDS_PAIRS_LIST | LADDS_PAIRS_DIGEST  */
   uint64_t log_capacity;             /* pair log entries kept on device by ladds_simulate     */
+  int32_t step_graph;                /* ladds_simulate with tpcd 1: replay each step from one  */
+                                     /* captured HIP graph (plane-tile scan, no profiling)     */
 } ladds_config;
 
 /* Object arrays, structure-of-arrays, host memory; copied in (caller keeps ownership). */

@@ -76,6 +76,7 @@ class EngineConfig:
     device: int = 0
     pair_output: int = PAIRS_LIST
     log_capacity: int = 1 << 20
+    step_graph: int = 0                   # replay each simulate() step from a captured HIP graph
 
     def to_c(self) -> _abi.Config:
         c = _abi.Config()

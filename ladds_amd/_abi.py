@@ -47,6 +47,7 @@ class Config(C.Structure):
         ("device", C.c_int32),
         ("pair_output", C.c_int32),
         ("log_capacity", C.c_uint64),
+        ("step_graph", C.c_int32),
     ]
 
 

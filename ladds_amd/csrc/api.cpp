@@ -105,6 +105,8 @@ struct ladds_ctx {
   int tune_threads = 0;
   int tune_tile = 0;
 
+  StepState *d_step = nullptr;  // graph replay position (ladds_config.step_graph)
+
   uint64_t step_count = 0;  // steps since t0
   double c[4], dcoef[3];
 
@@ -412,7 +414,8 @@ int ensure_log(ladds_ctx *ctx, uint64_t cap) {
   return LADDS_OK;
 }
 
-int launch_propagate_steps(ladds_ctx *ctx, size_t env_offset, int nsteps, bool bin, double burnup_r2) {
+int launch_propagate_steps(ladds_ctx *ctx, size_t env_offset, int nsteps, bool bin, double burnup_r2,
+                           const StepState *step_state = nullptr) {
   double cdt[4], ddt[3];
   for (int k = 0; k < 4; ++k) cdt[k] = ctx->c[k] * ctx->cfg.dt;
   for (int k = 0; k < 3; ++k) ddt[k] = ctx->dcoef[k] * ctx->cfg.dt;
@@ -421,14 +424,15 @@ int launch_propagate_steps(ladds_ctx *ctx, size_t env_offset, int nsteps, bool b
                             ctx->cfg.components, cdt, ddt, &ctx->grid,
                             (bin && ctx->variant == kVariantPlaneTiles) ? ctx->slot : nullptr,
                             (bin && ctx->variant == kVariantAtomicSort) ? ctx->count : nullptr, ctx->slot, burnup_r2,
-                            ctx->stream);
+                            step_state, ctx->stream);
   });
   if (e != hipSuccess) return ctx->hipfail(e, "propagate launch");
   return LADDS_OK;
 }
 
 // Binning (after the propagation's binning pass when `binned`) + pair scan on the current state.
-int launch_detection(ladds_ctx *ctx, bool binned, double dt_interp, bool digest, uint64_t iteration) {
+int launch_detection(ladds_ctx *ctx, bool binned, double dt_interp, bool digest, uint64_t iteration,
+                     const StepState *step_state = nullptr) {
   const FunctorParams fp = functor_params(ctx, dt_interp);
   hipError_t e = hipSuccess;
   if (ctx->variant == kVariantPlaneTiles) {
@@ -443,7 +447,7 @@ int launch_detection(ladds_ctx *ctx, bool binned, double dt_interp, bool digest,
       e = timed(ctx, "pair_tiles", [&] {
         return launch_pair_tiles(ctx->form, digest, ctx->d, ctx->grid, ctx->tiles, fp, ctx->rec, ctx->tps,
                                  ctx->scratch, ctx->arena_top, ctx->log, ctx->log_cap, ctx->log_count, ctx->counters,
-                                 iteration, ctx->stream);
+                                 iteration, step_state, ctx->stream);
       });
     if (e != hipSuccess) return ctx->hipfail(e, "detection launch");
     return LADDS_OK;
@@ -547,6 +551,7 @@ void ladds_config_default(ladds_config *cfg) {
   cfg->device = 0;
   cfg->pair_output = LADDS_PAIRS_LIST;
   cfg->log_capacity = 1u << 20;
+  cfg->step_graph = 0;
 }
 
 int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
@@ -618,6 +623,7 @@ int ladds_destroy(ladds_ctx *ctx) {
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   hipFree(ctx->d_tile_begin);
   hipFree(ctx->d_scen_tile);
+  hipFree(ctx->d_step);
   delete ctx;
   return LADDS_OK;
 }
@@ -998,9 +1004,53 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     const int rc = alloc_env(ctx);
     if (rc != LADDS_OK) return rc;
   }
+  // Graph replay: with a detection every step, one step is a fixed sequence of kernels
+  // (propagation + keys, partition, pair scan, step advance).  It is captured once into a
+  // HIP graph whose kernels read their position from the device StepState, and replayed
+  // once per step.  No memset is captured: the partition kernel resets the scratch arena.
+  const bool use_graph = ctx->cfg.step_graph != 0 && tpcd == 1 && ctx->variant == kVariantPlaneTiles &&
+                         !ctx->profiling && ctx->n > 0 && nsteps > 0;
+  hipGraphExec_t graph_exec = nullptr;
+  if (use_graph) {
+    if (!ctx->d_step) HIPCHK(ctx, dalloc(&ctx->d_step, 1));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    hipGraph_t graph = nullptr;
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_propagate_steps(ctx, 0, 1, true, burnup_r2, ctx->d_step);
+    if (rc == LADDS_OK) rc = launch_detection(ctx, true, ctx->cfg.dt, digest, 0, ctx->d_step);
+    hipError_t e = rc == LADDS_OK ? launch_advance_step(ctx->d_step, ctx->stream) : hipSuccess;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != LADDS_OK) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e == hipSuccess) e = ec;
+    if (e == hipSuccess) e = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
+    if (graph) hipGraphDestroy(graph);
+    if (e != hipSuccess) return ctx->hipfail(e, "step graph capture");
+  }
   for (int seg0 = 0; seg0 < nsteps && ctx->n > 0; seg0 += kEnvSegment) {
     const int seg = std::min(kEnvSegment, nsteps - seg0);
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // pinned environment buffer is free again
+    if (use_graph) {
+      hipError_t e = launch_set_step(ctx->d_step, 0u, first_iteration + (uint64_t)seg0, ctx->stream);
+      for (int staged = 0; staged < seg && e == hipSuccess;) {
+        const int k = std::min(kChunk, seg - staged);
+        const int rc = stage_env(ctx, staged, k);
+        if (rc != LADDS_OK) {
+          hipGraphExecDestroy(graph_exec);
+          return rc;
+        }
+        for (int q = 0; q < k && e == hipSuccess; ++q) e = hipGraphLaunch(graph_exec, ctx->stream);
+        staged += k;
+      }
+      if (e != hipSuccess) {
+        hipGraphExecDestroy(graph_exec);
+        return ctx->hipfail(e, "step graph launch");
+      }
+      ctx->step_count += (uint64_t)seg;
+      continue;
+    }
     int s = 0;
     int staged = 0;
     while (s < seg) {
@@ -1030,6 +1080,7 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   HIPCHK(ctx, hipMemcpyAsync(&sc, ctx->counters, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->log_count, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (graph_exec) hipGraphExecDestroy(graph_exec);
   if (min_altitude_km > 0.0) {
     const int rc = compact_if_needed(ctx);
     if (rc != LADDS_OK) return rc;

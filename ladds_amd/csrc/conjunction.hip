@@ -244,7 +244,7 @@ hipError_t launch_pair_scan(int form, bool digest, const DeviceObjects &o, const
                             ScanCounters *counters, uint64_t iteration, hipStream_t s) {
   if (o.n == 0) return hipSuccess;
   PairScanArgs a;
-  a.sink = PairSink{o, fp, log, log_cap, log_count, counters, iteration};
+  a.sink = PairSink{o, fp, log, log_cap, log_count, counters, iteration, nullptr};
   a.g = g;
   a.rec = rec;
   a.start = start;

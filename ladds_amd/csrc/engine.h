@@ -72,6 +72,14 @@ struct alignas(16) RecordF32 {
   uint32_t idx;
 };
 
+// Device-side step position for graph replay: a captured step reads which environment
+// and iteration it is at from here, and its last kernel advances `step`.
+struct StepState {
+  uint32_t step;      // step within the current environment segment
+  uint32_t pad;
+  uint64_t iter_base; // iteration of step 0 of the segment
+};
+
 // Device view of the object arrays.
 struct DeviceObjects {
   size_t n;

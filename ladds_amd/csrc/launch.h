@@ -12,7 +12,10 @@ hipError_t upload_atmosphere();
 hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<double> *env64,
                             const SubstepEnv<float> *env32, int nsteps, unsigned mask, const double cdt[4],
                             const double ddt[3], const GridParams *grid, uint32_t *keys, uint32_t *count,
-                            uint32_t *slot, double burnup_r2, hipStream_t stream);
+                            uint32_t *slot, double burnup_r2, const StepState *step_state, hipStream_t stream);
+// Graph replay bookkeeping: set the step position, advance it by one step.
+hipError_t launch_set_step(StepState *st, uint32_t step, uint64_t iter_base, hipStream_t stream);
+hipError_t launch_advance_step(StepState *st, hipStream_t stream);
 
 hipError_t launch_bin_count(int form, const DeviceObjects &o, const GridParams &g, uint32_t *count, uint32_t *slot,
                             hipStream_t s);
@@ -34,7 +37,7 @@ hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, cons
                              const TileParams &tp, const FunctorParams &fp, const void *items, const uint32_t *tps,
                              void *scratch, uint32_t *arena_top, LoggedPair *log, uint64_t log_cap,
                              unsigned long long *log_count, ScanCounters *counters, uint64_t iteration,
-                             hipStream_t s);
+                             const StepState *step_state, hipStream_t s);
 uint32_t pair_tiles_header_words(const TileParams &tp);
 hipError_t configure_tile_kernels();
 constexpr uint32_t kPairTileLdsBytes = 160 * 1024;

@@ -112,6 +112,7 @@ struct PairSink {
   unsigned long long *log_count;
   ScanCounters *counters;
   uint64_t iteration;
+  const StepState *step_state;  // non-null (graph replay): iteration = iter_base + step
 };
 
 struct ScanAcc {
@@ -160,7 +161,7 @@ __device__ __forceinline__ void handle_candidate(const PairSink &a, uint32_t oi,
         if (pos < a.log_cap) {
           LoggedPair lp;
           lp.pair = r;
-          lp.iteration = a.iteration;
+          lp.iteration = a.step_state ? a.step_state->iter_base + a.step_state->step : a.iteration;
           lp.evaded = (kind == 2) ? 1u : 0u;
           lp.scenario = scenario;
           a.log[pos] = lp;

@@ -35,6 +35,7 @@ struct PropagateArgs {
   uint32_t *count;  // non-null: counting pass of the atomic counting sort
   uint32_t *slot;
   double burnup_r2;  // > 0: objects with |r|^2 below become dummies (deleteBurnUps)
+  const StepState *step_state;  // non-null (graph replay): environments start at step_state->step
 };
 
 template <int FORM>
@@ -48,6 +49,7 @@ __global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
   }
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.o.n) return;
+  const size_t env0 = a.step_state ? 3u * (size_t)a.step_state->step : 0u;
   double x = a.o.rx[i], y = a.o.ry[i], z = a.o.rz[i];
   double vx = a.o.vx[i], vy = a.o.vy[i], vz = a.o.vz[i];
   ObjParams p;
@@ -71,8 +73,8 @@ __global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
       y += cdt * vy;
       z += cdt * vz;
       double ax, ay, az;
-      acceleration<FORM>(x, y, z, vx, vy, vz, p, a.mask, a.env64[3 * s + k], a.env32[3 * s + k], atm, ax, ay,
-                         az);
+      acceleration<FORM>(x, y, z, vx, vy, vz, p, a.mask, a.env64[env0 + 3 * s + k], a.env32[env0 + 3 * s + k],
+                         atm, ax, ay, az);
       vx += ddt * ax;
       vy += ddt * ay;
       vz += ddt * az;
@@ -112,7 +114,7 @@ hipError_t upload_atmosphere() {
 hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<double> *env64,
                             const SubstepEnv<float> *env32, int nsteps, unsigned mask, const double cdt[4],
                             const double ddt[3], const GridParams *grid, uint32_t *keys, uint32_t *count,
-                            uint32_t *slot, double burnup_r2, hipStream_t stream) {
+                            uint32_t *slot, double burnup_r2, const StepState *step_state, hipStream_t stream) {
   if (o.n == 0 || nsteps <= 0) return hipSuccess;
   PropagateArgs a;
   a.o = o;
@@ -127,6 +129,7 @@ hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<d
   a.count = count;
   a.slot = slot;
   a.burnup_r2 = burnup_r2;
+  a.step_state = step_state;
   const unsigned blocks = (unsigned)((o.n + 255) / 256);
   switch (form) {
     case kFormFp64:
@@ -139,6 +142,28 @@ hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<d
       hipLaunchKernelGGL(k_propagate<kFormSplitAll32>, dim3(blocks), dim3(256), 0, stream, a);
       break;
   }
+  return hipGetLastError();
+}
+
+__global__ void k_set_step(StepState *st, uint32_t step, uint64_t iter_base) {
+  if (threadIdx.x == 0) {
+    st->step = step;
+    st->pad = 0;
+    st->iter_base = iter_base;
+  }
+}
+
+__global__ void k_advance_step(StepState *st) {
+  if (threadIdx.x == 0) st->step += 1u;
+}
+
+hipError_t launch_set_step(StepState *st, uint32_t step, uint64_t iter_base, hipStream_t stream) {
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, stream, st, step, iter_base);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance_step(StepState *st, hipStream_t stream) {
+  hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, stream, st);
   return hipGetLastError();
 }
 

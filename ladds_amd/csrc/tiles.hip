@@ -488,10 +488,10 @@ hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, cons
                              const TileParams &tp, const FunctorParams &fp, const void *items, const uint32_t *tps,
                              void *scratch, uint32_t *arena_top, LoggedPair *log, uint64_t log_cap,
                              unsigned long long *log_count, ScanCounters *counters, uint64_t iteration,
-                             hipStream_t s) {
+                             const StepState *step_state, hipStream_t s) {
   if (o.n == 0) return hipSuccess;
   PairTileArgs a;
-  a.sink = PairSink{o, fp, log, log_cap, log_count, counters, iteration};
+  a.sink = PairSink{o, fp, log, log_cap, log_count, counters, iteration, step_state};
   a.g = g;
   a.tp = tp;
   a.items = items;

@@ -116,3 +116,43 @@ def test_bad_offsets_are_rejected():
         bad[2], bad[3] = bad[3], bad[2]
         with pytest.raises(la.LaddsError):
             eng.set_scenarios(bad)
+
+
+def _run(pop, off, graph, steps, rounds):
+    cfg = dict(dt=10.0, components=la.ALL_COMPONENTS, precision=la.SPLIT, cutoff_km=10.0,
+               threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, step_graph=graph)
+    out = []
+    with la.Engine(la.EngineConfig(**cfg)) as eng:
+        eng.set_objects(pop)
+        if off is not None:
+            eng.set_scenarios(off)
+        for k in range(rounds):
+            st = eng.simulate(steps, first_iteration=k * steps, tpcd=1, min_altitude_km=150.0)
+            log = eng.fetch_log()
+            eng.clear_log()
+            # eager work between the replays
+            hits, _ = eng.conjunctions()
+            r, v = eng.state()
+            keys = np.sort(np.array([(int(e["iteration"]), int(e["pair"]["id_a"]), int(e["pair"]["id_b"]),
+                                      int(e["scenario"])) for e in log], dtype=np.uint64).reshape(-1, 4), axis=0)
+            out.append((st["collisions"], st["evasions"], st["remaining"], keys, r, v, len(hits)))
+    return out
+
+
+@pytest.mark.parametrize("ensemble", [False, True])
+def test_graph_replay_matches_eager_launches(ensemble):
+    """Each simulate() call captures one step graph and replays it per step; three calls in a
+    row, with eager conjunction scans and synchronisations between them, must reproduce the
+    eagerly launched run exactly (pair logs with iterations, states, counters)."""
+    if ensemble:
+        pop, off = _ensemble()
+    else:
+        pop, off = la.generate_population(1, 0x1ADD5 + 21, 20000, planted_pairs=64), None
+    eager = _run(pop, off, 0, 30, 3)
+    graph = _run(pop, off, 1, 30, 3)
+    for a, b in zip(eager, graph):
+        assert a[:3] == b[:3]
+        assert np.array_equal(a[3], b[3])
+        assert np.array_equal(a[4], b[4]) and np.array_equal(a[5], b[5])
+        assert a[6] == b[6]
+    assert sum(x[0] for x in eager) > 0
