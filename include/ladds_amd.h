@@ -105,6 +105,7 @@ typedef struct ladds_config {
   uint64_t log_capacity;             /* pair log entries kept on device by ladds_simulate     */
   int32_t step_graph;                /* ladds_simulate with tpcd 1: replay each step from one  */
                                      /* captured HIP graph (plane-tile scan, no profiling)     */
+  int32_t scan_chunks;               /* plane-tile scan: workgroups per plane, 1-16 (0: auto)  */
 } ladds_config;
 
 /* Object arrays, structure-of-arrays, host memory; copied in (caller keeps ownership). */
@@ -220,9 +221,11 @@ LADDS_API int ladds_delete_burnups(ladds_ctx *ctx, double min_altitude_km, size_
  * The counterpart of AutoPas's tuning phase (autopas/tuning, driven by the
  * iteratePairwise calls of Simulation.cpp:211-222, whose return value reports
  * "still tuning"): time the plane-tile conjunction scan on the current state for
- * candidate cell sizes (1, 1.25, 1.5, 2 x the search radius), plane shapes (log2 buckets
- * per x-plane around the population-derived default), partition tile sizes (4096, 8192,
- * 16384 objects) and pair-kernel workgroup sizes, and keep the fastest for this context.  Pair results do not depend on the choice. */
+ * candidate workgroups per plane (1-16, for planes dense with clusters), pair-kernel
+ * workgroup sizes, cell sizes (1, 1.25, 1.5, 2 x the search radius), plane shapes (log2
+ * buckets per x-plane around the population-derived default) and partition tile sizes
+ * (4096, 8192, 16384 objects), one dimension at a time (coordinate descent), and keep the
+ * fastest for this context.  Pair results do not depend on the choice. */
 typedef struct ladds_tuning {
   double cell_size_km;   /* chosen cell side [km]                               */
   int32_t plane_bits;    /* chosen log2 buckets per x-plane                     */
@@ -231,6 +234,8 @@ typedef struct ladds_tuning {
   double default_us;     /* the same for the untuned default [us]               */
   int32_t tried;         /* configurations timed                                */
   int32_t tile_size;     /* chosen objects per partition tile                   */
+  int32_t chunks;        /* chosen workgroups per plane (dense planes)          */
+  int32_t reserved;
 } ladds_tuning;
 
 LADDS_API int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuning *out);

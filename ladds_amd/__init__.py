@@ -77,6 +77,7 @@ class EngineConfig:
     pair_output: int = PAIRS_LIST
     log_capacity: int = 1 << 20
     step_graph: int = 0                   # replay each simulate() step from a captured HIP graph
+    scan_chunks: int = 0                  # workgroups per plane of the plane-tile scan (0: auto)
 
     def to_c(self) -> _abi.Config:
         c = _abi.Config()
@@ -320,7 +321,7 @@ class Engine:
         dt = self.config.dt if dt_interp is None else dt_interp
         self._check(lib.ladds_autotune(self._h, float(dt), int(repetitions), C.byref(t)), "autotune")
         return {"cell_size_km": t.cell_size_km, "plane_bits": t.plane_bits, "pair_threads": t.pair_threads,
-                "tile_size": t.tile_size, "scan_us": t.scan_us, "default_us": t.default_us, "tried": t.tried}
+                "tile_size": t.tile_size, "chunks": t.chunks, "scan_us": t.scan_us, "default_us": t.default_us, "tried": t.tried}
 
     # -- measurement ----------------------------------------------------
     def set_profiling(self, enable: bool):

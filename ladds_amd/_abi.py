@@ -48,6 +48,7 @@ class Config(C.Structure):
         ("pair_output", C.c_int32),
         ("log_capacity", C.c_uint64),
         ("step_graph", C.c_int32),
+        ("scan_chunks", C.c_int32),
     ]
 
 
@@ -105,6 +106,8 @@ class Tuning(C.Structure):
         ("default_us", C.c_double),
         ("tried", C.c_int32),
         ("tile_size", C.c_int32),
+        ("chunks", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "../../include/ladds_amd.h"
@@ -75,6 +76,7 @@ struct ladds_ctx {
   void *rec = nullptr;
   uint32_t *tps = nullptr, *arena_top = nullptr;
   void *scratch = nullptr;
+  size_t scratch_bytes = 0;
   size_t tps_alloc = 0;
   uint32_t *n_alive = nullptr;
   uint32_t grid_buckets_alloc = 0;
@@ -104,6 +106,7 @@ struct ladds_ctx {
   int tune_pb = -1;
   int tune_threads = 0;
   int tune_tile = 0;
+  int tune_chunks = 0;
 
   StepState *d_step = nullptr;  // graph replay position (ladds_config.step_graph)
 
@@ -342,9 +345,26 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
       for (size_t p = 0; p < P; ++p)
         pair_max = std::max<size_t>(pair_max, plane_count[sc * P + p] + plane_count[sc * P + (p + 1) % P]);
   }
+  // dense planes (clusters): several workgroups per plane
+  size_t own_max = 0;
+  for (uint32_t c : plane_count) own_max = std::max<size_t>(own_max, c);
+  tp.nchunk = (uint32_t)std::min<size_t>(16, std::max<size_t>(1, own_max / (4 * (size_t)tp.pair_threads)));
+  if (ctx->cfg.scan_chunks >= 1 && ctx->cfg.scan_chunks <= 16) tp.nchunk = (uint32_t)ctx->cfg.scan_chunks;
+  if (ctx->tune_chunks >= 1 && ctx->tune_chunks <= 16) tp.nchunk = (uint32_t)ctx->tune_chunks;
   const size_t want = std::max<size_t>(1024, ((pair_max + pair_max / 4) + 255) / 256 * 256);
   const size_t fit = (kPairTileLdsBytes - 4 * (size_t)tp.lds_header_words) / rec;
   tp.cap_items = (uint32_t)std::min(want, fit);
+  // scratch arena for planes beyond the LDS capacity: every chunk of a plane sorts it
+  const size_t arena_need = std::max<size_t>(2 * n, 1024) * tp.nchunk;
+  const size_t arena_bytes = arena_need * sizeof(RecordF64);
+  if (tiles && arena_bytes > ctx->scratch_bytes) {
+    hipFree(ctx->scratch);
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->scratch, arena_bytes));
+    ctx->scratch_bytes = arena_bytes;
+  }
+  tp.arena_cap = (uint32_t)std::min<size_t>(ctx->scratch_bytes / rec, 0xFFFFFFFFu);
   ctx->tiles = tp;
   const size_t tps_need = (size_t)tp.ntiles * (tp.nplanes + 1);
   if (tps_need > ctx->tps_alloc) {
@@ -552,6 +572,7 @@ void ladds_config_default(ladds_config *cfg) {
   cfg->pair_output = LADDS_PAIRS_LIST;
   cfg->log_capacity = 1u << 20;
   cfg->step_graph = 0;
+  cfg->scan_chunks = 0;
 }
 
 int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
@@ -671,8 +692,9 @@ int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
     HIPCHK(ctx, alloc_objects(ctx->d, n));
     HIPCHK(ctx, dalloc(&ctx->slot, n));
     hipFree(ctx->scratch);
+    ctx->scratch = nullptr;  // the scan's scratch arena is sized by setup_grid
+    ctx->scratch_bytes = 0;
     HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&ctx->rec), n));
-    HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&ctx->scratch), 2 * n));
     ctx->cap = n;
   }
   {
@@ -828,6 +850,7 @@ static int run_detection_sync(ladds_ctx *ctx, double dt_interp, bool digest, Sca
   HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->log_count, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   *entries = cnt;
+  if (out_counters->overflow) return ctx->fail(LADDS_E_OVERFLOW, "conjunction scan scratch arena exhausted");
   return LADDS_OK;
 }
 
@@ -841,6 +864,7 @@ static int time_detection(ladds_ctx *ctx, double dt_interp, int reps, double *us
   int rc = LADDS_OK;
   for (int r = 0; r <= reps && rc == LADDS_OK; ++r) {
     (void)hipEventRecord(e0, ctx->stream);
+    HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
     rc = launch_detection(ctx, false, dt_interp, true, 0);
     (void)hipEventRecord(e1, ctx->stream);
     if (hipEventSynchronize(e1) != hipSuccess) rc = ctx->fail(LADDS_E_HIP, "detection timing failed");
@@ -861,54 +885,70 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   const size_t n = ctx->n;
   std::vector<double> rx(n);
   HIPCHK(ctx, hipMemcpy(rx.data(), ctx->d.rx, sizeof(double) * n, hipMemcpyDeviceToHost));
-  // the untuned default first
+  // start from the untuned default
   ctx->tune_cell_km = 0.0;
   ctx->tune_pb = -1;
   ctx->tune_threads = 0;
   ctx->tune_tile = 0;
+  ctx->tune_chunks = 0;
   int rc = setup_grid(ctx, n, rx.data());
   if (rc != LADDS_OK) return rc;
   const double base_cell = 1.0 / ctx->grid.inv_cell;
   const int base_pb = (int)(ctx->grid.by + ctx->grid.bz);
+  // current point of the search: {cell, plane bits, threads, tile, chunks}
+  double cur_cell = base_cell;
+  int cur[4] = {base_pb, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk};
   double default_us = 0.0;
   rc = time_detection(ctx, dt_interp, repetitions, &default_us);
   if (rc != LADDS_OK) return rc;
-  double best_us = default_us, best_cell = 0.0;
-  int best_pb = -1, best_threads = 0, tried = 1;
-  const double cell_factors[] = {1.0, 1.25, 1.5, 2.0};
-  const int threads[] = {128, 256, 512, 1024};
-  const int tiles[] = {4096, 8192, 16384};
-  int best_tile = 0;
-  for (double f : cell_factors) {
-    for (int dpb = -1; dpb <= 2; ++dpb) {
-      const int pb = base_pb + dpb;
-      if (pb < 4 || pb > 12) continue;
-      for (int t : threads) for (int tile : tiles) {
-        ctx->tune_cell_km = base_cell * f;
-        ctx->tune_pb = pb;
-        ctx->tune_threads = t;
-        ctx->tune_tile = tile;
-        rc = setup_grid(ctx, n, rx.data());
-        if (rc != LADDS_OK) return rc;
-        double us = 0.0;
-        rc = time_detection(ctx, dt_interp, repetitions, &us);
-        if (rc != LADDS_OK) return rc;
-        ++tried;
-        if (us < best_us) {
-          best_us = us;
-          best_cell = ctx->tune_cell_km;
-          best_pb = pb;
-          best_threads = t;
-          best_tile = tile;
-        }
+  double best_us = default_us;
+  int tried = 1;
+  auto apply = [&](double cell, const int *v) {
+    ctx->tune_cell_km = cell;
+    ctx->tune_pb = v[0];
+    ctx->tune_threads = v[1];
+    ctx->tune_tile = v[2];
+    ctx->tune_chunks = v[3];
+    return setup_grid(ctx, n, rx.data());
+  };
+  // Coordinate descent, one dimension at a time, keeping every improvement: chunks per
+  // plane (load balance of dense planes), workgroup size, cell size, plane shape, tile.
+  const int chunk_opts[] = {1, 2, 4, 8, 16};
+  const int thread_opts[] = {128, 256, 512, 1024};
+  const double cell_opts[] = {1.0, 1.25, 1.5, 2.0};
+  const int tile_opts[] = {4096, 8192, 16384};
+  for (int dim = 0; dim < 5; ++dim) {
+    std::vector<std::pair<double, std::array<int, 4>>> cands;
+    std::array<int, 4> c{cur[0], cur[1], cur[2], cur[3]};
+    if (dim == 0)
+      for (int k : chunk_opts) { c[3] = k; cands.push_back({cur_cell, c}); }
+    if (dim == 1)
+      for (int t : thread_opts) { c[1] = t; cands.push_back({cur_cell, c}); }
+    if (dim == 2)
+      for (double f : cell_opts) cands.push_back({base_cell * f, c});
+    if (dim == 3)
+      for (int d = -1; d <= 2; ++d)
+        if (base_pb + d >= 4 && base_pb + d <= 12) { c[0] = base_pb + d; cands.push_back({cur_cell, c}); }
+    if (dim == 4)
+      for (int t : tile_opts) { c[2] = t; cands.push_back({cur_cell, c}); }
+    for (const auto &cand : cands) {
+      if (cand.first == cur_cell && cand.second[0] == cur[0] && cand.second[1] == cur[1] &&
+          cand.second[2] == cur[2] && cand.second[3] == cur[3])
+        continue;
+      rc = apply(cand.first, cand.second.data());
+      if (rc != LADDS_OK) return rc;
+      double us = 0.0;
+      rc = time_detection(ctx, dt_interp, repetitions, &us);
+      if (rc != LADDS_OK) return rc;
+      ++tried;
+      if (us < best_us) {
+        best_us = us;
+        cur_cell = cand.first;
+        for (int k = 0; k < 4; ++k) cur[k] = cand.second[k];
       }
     }
   }
-  ctx->tune_cell_km = best_cell;
-  ctx->tune_pb = best_pb;
-  ctx->tune_threads = best_threads;
-  ctx->tune_tile = best_tile;
-  rc = setup_grid(ctx, n, rx.data());
+  rc = apply(cur_cell, cur);
   if (rc != LADDS_OK) return rc;
   HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
@@ -918,6 +958,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     out->plane_bits = (int32_t)(ctx->grid.by + ctx->grid.bz);
     out->pair_threads = (int32_t)ctx->tiles.pair_threads;
     out->tile_size = (int32_t)ctx->tiles.tile_size;
+    out->chunks = (int32_t)ctx->tiles.nchunk;
     out->scan_us = best_us;
     out->default_us = default_us;
     out->tried = tried;
@@ -1081,6 +1122,7 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->log_count, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (graph_exec) hipGraphExecDestroy(graph_exec);
+  if (sc.overflow) return ctx->fail(LADDS_E_OVERFLOW, "conjunction scan scratch arena exhausted");
   if (min_altitude_km > 0.0) {
     const int rc = compact_if_needed(ctx);
     if (rc != LADDS_OK) return rc;

@@ -58,6 +58,10 @@ struct TileParams {
   uint32_t max_scen_tiles;     // most tiles of one scenario (sizes the pair kernel's LDS)
   const uint32_t *tile_begin;  // nscen > 1: first object of every tile (ntiles + 1 entries)
   const uint32_t *scen_tile;   // nscen > 1: first tile of every scenario (nscen + 1 entries)
+  // Dense planes: each plane is scanned by nchunk workgroups, chunk c taking the plane's
+  // objects with index = c (mod nchunk); every chunk sorts the plane itself.
+  uint32_t nchunk;
+  uint32_t arena_cap;          // records of the global scratch arena
 };
 
 // Sorted record of the plain FP64 form and of the split form: the position the
@@ -107,7 +111,8 @@ struct LoggedPair {  // same layout as ladds_logged_pair
 
 // Counters written by the pair scan (device memory).
 struct ScanCounters {
-  unsigned long long hits, evaded, candidates, pad;
+  unsigned long long hits, evaded, candidates;
+  unsigned long long overflow;  // planes skipped because the scratch arena was exhausted
   unsigned long long digest[2], edigest[2];
 };
 

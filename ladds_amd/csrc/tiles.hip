@@ -205,13 +205,17 @@ __device__ __forceinline__ uint32_t bucket_begin(const uint32_t *cur, uint32_t b
 template <int FORM, bool DIGEST>
 __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FORM> *own, const RecOf<FORM> *halo,
                                            const uint32_t *cur_own, const uint32_t *cur_halo, uint32_t n_own,
-                                           uint32_t threads, uint32_t scen, ScanAcc &acc) {
+                                           uint32_t threads, uint32_t scen, uint32_t chunk, uint32_t nchunk,
+                                           ScanAcc &acc) {
   const GridParams &g = a.g;
   const uint32_t my = (1u << g.by) - 1u, mz = (1u << g.bz) - 1u;
   const uint32_t sy = g.bz;
 #pragma unroll 1
   for (uint32_t s = threadIdx.x; s < n_own; s += threads) {
     const RecOf<FORM> me = own[s];
+    // chunks split the plane's objects by object index: the placement order differs
+    // between the chunks' own sorts, the index does not
+    if (nchunk > 1u && me.idx % nchunk != chunk) continue;
     double x, y, z;
     binning_point_of(me, x, y, z);
     const uint32_t lb = cell_of(x, y, z, g).bucket;
@@ -221,7 +225,12 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
       const bool next_plane = (kind & 1u) != 0u;
       const RecOf<FORM> *arr = next_plane ? halo : own;
       const uint32_t *cur = next_plane ? cur_halo : cur_own;
-      const uint32_t from = next_plane ? 0u : s + 1u;
+      // Own-plane pairs: with one workgroup per plane each pair is taken from its lower
+      // sorted position.  The in-LDS placement order within a bucket depends on atomic
+      // timing, so chunks of one plane (each sorting the plane itself) cannot share that
+      // rule: they take a pair from its lower object index instead.
+      const bool by_index = !next_plane && nchunk > 1u;
+      const uint32_t from = (next_plane || by_index) ? 0u : s + 1u;
       const uint32_t row = ((fy + (kind >> 1) - 1u) & my) << sy;
       uint32_t lo0, hi0, lo1 = 0u, hi1 = 0u;
       if (fz != 0u && fz != mz) {
@@ -244,7 +253,8 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
 #pragma unroll 1
         for (; q < end; ++q) {
           const RecOf<FORM> other = arr[q];
-          if (within<FORM>(me, other, a.sink.fp)) handle_candidate<DIGEST>(a.sink, me.idx, other.idx, acc, scen);
+          if ((!by_index || other.idx > me.idx) && within<FORM>(me, other, a.sink.fp))
+            handle_candidate<DIGEST>(a.sink, me.idx, other.idx, acc, scen);
         }
         q = max(lo1, from);
         end = hi1;
@@ -260,9 +270,10 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
   extern __shared__ uint32_t lds[];
   const TileParams &tp = a.tp;
   const uint32_t P = tp.nplanes, L = 1u << tp.plane_bits;
-  // workgroup = (scenario, plane); a scenario's tiles are [t0, t0 + T)
-  const uint32_t scen = blockIdx.x / P;
-  const uint32_t plane = blockIdx.x - scen * P, next = (plane + 1u) & (P - 1u);  // P >= 4
+  // workgroup = (scenario, plane, chunk); a scenario's tiles are [t0, t0 + T)
+  const uint32_t chunk = blockIdx.x % tp.nchunk, pidx = blockIdx.x / tp.nchunk;
+  const uint32_t scen = pidx / P;
+  const uint32_t plane = pidx - scen * P, next = (plane + 1u) & (P - 1u);  // P >= 4
   const uint32_t t0 = tp.scen_tile ? tp.scen_tile[scen] : 0u;
   const uint32_t T = tp.scen_tile ? tp.scen_tile[scen + 1] - t0 : tp.ntiles;
   // LDS layout
@@ -303,8 +314,17 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
   if (n_own != 0u) {
     const uint32_t total = n_own + n_next;
     const bool in_lds = total <= tp.cap_items;
-    if (!in_lds && threadIdx.x == 0) misc[0] = atomicAdd(a.arena_top, total);
+    if (!in_lds && threadIdx.x == 0) {
+      const uint32_t off = atomicAdd(a.arena_top, total);
+      misc[0] = off;
+      misc[1] = (off <= tp.arena_cap && total <= tp.arena_cap - off) ? 1u : 0u;
+      if (!misc[1]) atomicAdd(&a.sink.counters->overflow, 1ull);
+    }
     __syncthreads();
+    if (!in_lds && !misc[1]) {  // no room: the plane is reported, never written out of bounds
+      flush_counters<DIGEST>(a.sink, acc);
+      return;
+    }
     REC *sorted = in_lds ? lds_items : static_cast<REC *>(a.scratch) + misc[0];
     REC *own = sorted, *halo = sorted + n_own;
     const uint32_t mloc = L - 1u;
@@ -384,9 +404,10 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
     }
     // cur[b] now holds the end of bucket b
     if (in_lds)
-      scan_plane<FORM, DIGEST>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, THREADS, scen, acc);
+      scan_plane<FORM, DIGEST>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, THREADS, scen, chunk,
+                               tp.nchunk, acc);
     else
-      scan_plane<FORM, DIGEST>(a, own, halo, cur_own, cur_next, n_own, THREADS, scen, acc);
+      scan_plane<FORM, DIGEST>(a, own, halo, cur_own, cur_next, n_own, THREADS, scen, chunk, tp.nchunk, acc);
   }
   flush_counters<DIGEST>(a.sink, acc);
 }
@@ -476,7 +497,7 @@ uint32_t pair_tiles_header_words(const TileParams &tp) {
 
 template <int THREADS>
 void launch_pair_tiles_as(int form, bool digest, const PairTileArgs &a, size_t shm, hipStream_t s) {
-  const dim3 grid(a.tp.nplanes * a.tp.nscen), block(THREADS);
+  const dim3 grid(a.tp.nplanes * a.tp.nscen * a.tp.nchunk), block(THREADS);
   const bool f64 = (form == kFormFp64);
   if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false, THREADS>), grid, block, shm, s, a);
   if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true, THREADS>), grid, block, shm, s, a);

@@ -36,11 +36,11 @@ def _key(p, shift=0):
                    float(x["d2"]).hex(), tuple(float(m).hex() for m in x["mid"])) for x in p)
 
 
-@pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
-def test_pairs_per_scenario(precision):
+@pytest.mark.parametrize("precision,chunks", [(la.FP64, 0), (la.SPLIT, 0), (la.SPLIT, 3)])
+def test_pairs_per_scenario(precision, chunks):
     pop, off = _ensemble()
     cfg = dict(dt=10.0, components=la.KEP | la.J2, precision=precision, cutoff_km=10.0,
-               threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0)
+               threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, scan_chunks=chunks)
     with la.Engine(la.EngineConfig(**cfg)) as eng:
         eng.set_objects(pop)
         eng.set_scenarios(off)

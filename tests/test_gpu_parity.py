@@ -408,3 +408,25 @@ def test_autotune_keeps_the_pair_set(precision):
     assert oh.shape[0] > 100
     _pair_equal(ol.sort_pairs(hits), oh)
     _pair_equal(ol.sort_pairs(ev), oe)
+
+
+@pytest.mark.parametrize("chunks", [2, 3, 16])
+@pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
+def test_chunked_planes_keep_the_pair_set(chunks, precision):
+    """Several workgroups per plane (each sorting the plane itself) still report every pair
+    exactly once, including a dense cell where every object is close to every other."""
+    pop = la.generate_population(1, 0x1ADD5 + 13, 30000, planted_pairs=200)
+    c = np.array([pop.rx[0], pop.ry[0], pop.rz[0]])
+    rng = np.random.default_rng(13)
+    for i in range(1, 400):  # a dense cluster of 400 objects within ~0.3 km
+        pop.rx[i], pop.ry[i], pop.rz[i] = c + rng.uniform(-0.15, 0.15, 3)
+        pop.vx[i], pop.vy[i], pop.vz[i] = pop.vx[0], pop.vy[0], pop.vz[0] + rng.uniform(-1e-3, 1e-3)
+    with engine(dt=10.0, precision=precision, cutoff_km=10.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
+                threshold_km=1.0, scan_chunks=chunks) as eng:
+        eng.set_objects(pop)
+        hits, ev = eng.conjunctions()
+    objs = _oracle_objects(pop.r(), pop.v(), pop)
+    oh, oe = ol.pairs(objs, ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0))
+    assert oh.shape[0] > 400 * 399 // 2
+    _pair_equal(ol.sort_pairs(hits), oh)
+    _pair_equal(ol.sort_pairs(ev), oe)
