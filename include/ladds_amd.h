@@ -105,7 +105,7 @@ typedef struct ladds_config {
   uint64_t log_capacity;             /* pair log entries kept on device by ladds_simulate     */
   int32_t step_graph;                /* ladds_simulate with tpcd 1: replay each step from one  */
                                      /* captured HIP graph (plane-tile scan, no profiling)     */
-  int32_t scan_chunks;               /* plane-tile scan: workgroups per plane, 1-16 (0: auto)  */
+  int32_t scan_chunks;               /* plane-tile scan: most workgroups per plane, 1-32 (0: auto) */
 } ladds_config;
 
 /* Object arrays, structure-of-arrays, host memory; copied in (caller keeps ownership). */
@@ -221,7 +221,7 @@ LADDS_API int ladds_delete_burnups(ladds_ctx *ctx, double min_altitude_km, size_
  * The counterpart of AutoPas's tuning phase (autopas/tuning, driven by the
  * iteratePairwise calls of Simulation.cpp:211-222, whose return value reports
  * "still tuning"): time the plane-tile conjunction scan on the current state for
- * candidate workgroups per plane (1-16, for planes dense with clusters), pair-kernel
+ * candidate workgroups per plane (1-32, for planes dense with clusters), pair-kernel
  * workgroup sizes, cell sizes (1, 1.25, 1.5, 2 x the search radius), plane shapes (log2
  * buckets per x-plane around the population-derived default) and partition tile sizes
  * (4096, 8192, 16384 objects), one dimension at a time (coordinate descent), and keep the

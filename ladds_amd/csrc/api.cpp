@@ -349,8 +349,8 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   size_t own_max = 0;
   for (uint32_t c : plane_count) own_max = std::max<size_t>(own_max, c);
   tp.nchunk = (uint32_t)std::min<size_t>(16, std::max<size_t>(1, own_max / (4 * (size_t)tp.pair_threads)));
-  if (ctx->cfg.scan_chunks >= 1 && ctx->cfg.scan_chunks <= 16) tp.nchunk = (uint32_t)ctx->cfg.scan_chunks;
-  if (ctx->tune_chunks >= 1 && ctx->tune_chunks <= 16) tp.nchunk = (uint32_t)ctx->tune_chunks;
+  if (ctx->cfg.scan_chunks >= 1 && ctx->cfg.scan_chunks <= 32) tp.nchunk = (uint32_t)ctx->cfg.scan_chunks;
+  if (ctx->tune_chunks >= 1 && ctx->tune_chunks <= 32) tp.nchunk = (uint32_t)ctx->tune_chunks;
   const size_t want = std::max<size_t>(1024, ((pair_max + pair_max / 4) + 255) / 256 * 256);
   const size_t fit = (kPairTileLdsBytes - 4 * (size_t)tp.lds_header_words) / rec;
   tp.cap_items = (uint32_t)std::min(want, fit);
@@ -913,7 +913,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   };
   // Coordinate descent, one dimension at a time, keeping every improvement: chunks per
   // plane (load balance of dense planes), workgroup size, cell size, plane shape, tile.
-  const int chunk_opts[] = {1, 2, 4, 8, 16};
+  const int chunk_opts[] = {1, 2, 4, 8, 16, 32};
   const int thread_opts[] = {128, 256, 512, 1024};
   const double cell_opts[] = {1.0, 1.25, 1.5, 2.0};
   const int tile_opts[] = {4096, 8192, 16384};

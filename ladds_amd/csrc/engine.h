@@ -58,8 +58,9 @@ struct TileParams {
   uint32_t max_scen_tiles;     // most tiles of one scenario (sizes the pair kernel's LDS)
   const uint32_t *tile_begin;  // nscen > 1: first object of every tile (ntiles + 1 entries)
   const uint32_t *scen_tile;   // nscen > 1: first tile of every scenario (nscen + 1 entries)
-  // Dense planes: each plane is scanned by nchunk workgroups, chunk c taking the plane's
-  // objects with index = c (mod nchunk); every chunk sorts the plane itself.
+  // Dense planes: a plane of n_own objects is scanned by min(nchunk, ceil(n_own / 512))
+  // workgroups, chunk c taking the objects with index = c (mod chunks); every chunk sorts
+  // the plane itself (grid: nchunk workgroups per plane, the unused ones exit).
   uint32_t nchunk;
   uint32_t arena_cap;          // records of the global scratch arena
 };

@@ -36,6 +36,7 @@ constexpr int kPartThreads = 1024;
 // the latency-bound scan more waves when only one fits (the FP64 records are twice the size).
 constexpr int kMaxPairThreads = 1024;
 constexpr int kPairRegCapacity = 2048;  // records kept in registers between the two sort passes (at most)
+constexpr uint32_t kChunkRecords = 512;  // own records per chunk of a dense plane
 
 // In-place exclusive scan of a[0..n) in LDS by the whole block; returns the total.
 // `wsum` holds blockDim/64 + 1 entries.
@@ -311,7 +312,9 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
     seg_next[T] = n_next;
   }
   ScanAcc acc;
-  if (n_own != 0u) {
+  // chunks this plane uses: ~kChunkRecords own records each, at most tp.nchunk
+  const uint32_t cp = min(tp.nchunk, max(1u, (n_own + kChunkRecords - 1u) / kChunkRecords));
+  if (n_own != 0u && chunk < cp) {
     const uint32_t total = n_own + n_next;
     const bool in_lds = total <= tp.cap_items;
     if (!in_lds && threadIdx.x == 0) {
@@ -380,19 +383,32 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
       __syncthreads();
     } else {
       // pass 1: bucket histogram; pass 2: placement (records read twice)
+      // Records are fetched kBatch at a time per thread so that their locations (LDS
+      // searches) and global loads overlap instead of forming one dependent chain.
+      constexpr uint32_t kBatch = 4;
 #pragma unroll 1
       for (int pass = 0; pass < 2; ++pass) {
 #pragma unroll 1
-        for (uint32_t j = threadIdx.x; j < total; j += blockDim.x) {
-          const bool is_own = j < n_own;
-          const REC r = items[loc(j)];
-          uint32_t *cur = is_own ? cur_own : cur_next;
-          const uint32_t b = local_bucket(r);
-          if (pass == 0) {
-            atomicAdd(&cur[b], 1u);
-          } else {
-            const uint32_t pos = atomicAdd(&cur[b], 1u);
-            (is_own ? own : halo)[pos] = r;
+        for (uint32_t j0 = threadIdx.x; j0 < total; j0 += kBatch * THREADS) {
+          REC r[kBatch];
+#pragma unroll
+          for (uint32_t u = 0; u < kBatch; ++u) {
+            const uint32_t j = j0 + u * THREADS;
+            if (j < total) r[u] = items[loc(j)];
+          }
+#pragma unroll
+          for (uint32_t u = 0; u < kBatch; ++u) {
+            const uint32_t j = j0 + u * THREADS;
+            if (j >= total) continue;
+            const bool is_own = j < n_own;
+            uint32_t *cur = is_own ? cur_own : cur_next;
+            const uint32_t b = local_bucket(r[u]);
+            if (pass == 0) {
+              atomicAdd(&cur[b], 1u);
+            } else {
+              const uint32_t pos = atomicAdd(&cur[b], 1u);
+              (is_own ? own : halo)[pos] = r[u];
+            }
           }
         }
         __syncthreads();
@@ -405,9 +421,9 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
     // cur[b] now holds the end of bucket b
     if (in_lds)
       scan_plane<FORM, DIGEST>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, THREADS, scen, chunk,
-                               tp.nchunk, acc);
+                               cp, acc);
     else
-      scan_plane<FORM, DIGEST>(a, own, halo, cur_own, cur_next, n_own, THREADS, scen, chunk, tp.nchunk, acc);
+      scan_plane<FORM, DIGEST>(a, own, halo, cur_own, cur_next, n_own, THREADS, scen, chunk, cp, acc);
   }
   flush_counters<DIGEST>(a.sink, acc);
 }

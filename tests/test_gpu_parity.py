@@ -418,7 +418,7 @@ def test_chunked_planes_keep_the_pair_set(chunks, precision):
     pop = la.generate_population(1, 0x1ADD5 + 13, 30000, planted_pairs=200)
     c = np.array([pop.rx[0], pop.ry[0], pop.rz[0]])
     rng = np.random.default_rng(13)
-    for i in range(1, 400):  # a dense cluster of 400 objects within ~0.3 km
+    for i in range(1, 1500):  # a dense cluster of 1500 objects within ~0.3 km (three chunks of 512)
         pop.rx[i], pop.ry[i], pop.rz[i] = c + rng.uniform(-0.15, 0.15, 3)
         pop.vx[i], pop.vy[i], pop.vz[i] = pop.vx[0], pop.vy[0], pop.vz[0] + rng.uniform(-1e-3, 1e-3)
     with engine(dt=10.0, precision=precision, cutoff_km=10.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
@@ -427,6 +427,6 @@ def test_chunked_planes_keep_the_pair_set(chunks, precision):
         hits, ev = eng.conjunctions()
     objs = _oracle_objects(pop.r(), pop.v(), pop)
     oh, oe = ol.pairs(objs, ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0))
-    assert oh.shape[0] > 400 * 399 // 2
+    assert oh.shape[0] > 1500 * 1499 // 2
     _pair_equal(ol.sort_pairs(hits), oh)
     _pair_equal(ol.sort_pairs(ev), oe)
