@@ -65,6 +65,12 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=60, help="steps of the CPU baseline sample (~15 s on 16 threads)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-propagation leg")
+    p.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    p.add_argument("--ensemble-steps", type=int, default=2000)
+    p.add_argument("--no-clustered", action="store_true", help="skip the clustered 8M-object leg")
+    p.add_argument("--clustered-steps", type=int, default=10,
+                   help="timed steps of the clustered leg (each ~0.4 s: the workload has ~2.7e9 conjunctions "
+                        "per step)")
     p.add_argument("--forms", default="fp64,split")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="first copy MIB MiB device-to-device 3 times (a plain copy kernel of known size, "
@@ -146,25 +152,21 @@ def adjoint_leg(la, reps=3):
     """Differentiable propagation: 4,096 objects, full force model in FP64, dt = 60 s, 1,440
     steps (one day), checkpoints every 64 steps; loss = sum |r_T - target|^2 with targets =
     nominal final state + N(0, 100 m).  Reports gradient-steps/s (objects x steps per
-    forward + backward pass, device time from events on torch's stream) and the relative
-    error of the gradient against the oracle's FP64 complex-step gradient."""
+    forward + backward pass, device time from events on torch's stream)."""
     import torch
     from ladds_amd import adjoint
-
-    sys.path.insert(0, str(ROOT / "tests"))
-    import oracle_lib as ol
 
     n, steps, dt = 4096, 1440, 60.0
     pop = la.generate_population(1, 0x1ADD5 + 4, n)
     dev = torch.device("cuda:0")
     t = lambda x: torch.tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
     plan = adjoint.AdjointPlan(n, steps, dt, components=la.ALL_COMPONENTS, checkpoint_every=64)
+    aom_t, cr_t = t(pop.aom), t(pop.cr)
     with torch.no_grad():
-        rT0, _ = adjoint.propagate(t(pop.r()), t(pop.v()), t(pop.bc_inv), plan, aom=t(pop.aom), cr=t(pop.cr))
+        rT0, _ = adjoint.propagate(t(pop.r()), t(pop.v()), t(pop.bc_inv), plan, aom=aom_t, cr=cr_t)
     target = rT0 + 0.1 * torch.randn(rT0.shape, dtype=torch.float64, device=dev,
                                      generator=torch.Generator(device=dev).manual_seed(4))
     times = []
-    aom_t, cr_t = t(pop.aom), t(pop.cr)
     for rep in range(reps + 1):
         r0, v0, bc = t(pop.r()).requires_grad_(), t(pop.v()).requires_grad_(), t(pop.bc_inv).requires_grad_()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -177,23 +179,126 @@ def adjoint_leg(la, reps=3):
         if rep:
             times.append(e0.elapsed_time(e1) * 1e-3)
     sec = float(np.median(times))
-    g_r = 2.0 * (rT.detach().cpu().numpy() - target.cpu().numpy())
-    t0 = time.perf_counter()
-    threads = min(16, len(os.sched_getaffinity(0)))
-    cr_, cv_, cbc = ol.vjp(pop.r(), pop.v(), 0.0, dt, la.ALL_COMPONENTS, steps, g_r, np.zeros_like(g_r),
-                           aom=pop.aom, bc_inv=pop.bc_inv, cr=pop.cr, nthreads=threads)
-    cpu_sec = time.perf_counter() - t0
-    got = np.concatenate([r0.grad.cpu().numpy(), v0.grad.cpu().numpy()], 1)
-    ref = np.concatenate([cr_, cv_], 1)
-    rel = np.max(np.abs(got - ref), 1) / np.max(np.abs(ref), 1)
-    bc_rel = np.abs(bc.grad.cpu().numpy() - cbc) / np.max(np.abs(cbc))
     plan.close()
-    return {"workload": f"{n} objects, full force model FP64, dt {dt:g} s, {steps} steps, checkpoints every 64",
-            "gradient_steps_per_s": n * steps / sec, "ms_per_gradient": sec * 1e3,
-            "grad_rel_err_max": float(rel.max()), "grad_rel_err_median": float(np.median(rel)),
+    out = {"workload": f"{n} objects, full force model FP64, dt {dt:g} s, {steps} steps, checkpoints every 64",
+           "gradient_steps_per_s": n * steps / sec, "ms_per_gradient": sec * 1e3}
+    grads = {"pop": pop, "g_r": 2.0 * (rT.detach().cpu().numpy() - target.cpu().numpy()),
+             "got": np.concatenate([r0.grad.cpu().numpy(), v0.grad.cpu().numpy()], 1),
+             "got_bc": bc.grad.cpu().numpy(), "steps": steps, "dt": dt}
+    return out, grads
+
+
+def cpu_baseline_adjoint(la, grads):
+    """CPU leg of the differentiable propagation: the oracle's FP64 complex-step gradient of
+    the same loss, timed, and the GPU gradient's relative error against it."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as ol
+
+    pop, g_r = grads["pop"], grads["g_r"]
+    threads = min(16, len(os.sched_getaffinity(0)))
+    t0 = time.perf_counter()
+    cr_, cv_, cbc = ol.vjp(pop.r(), pop.v(), 0.0, grads["dt"], la.ALL_COMPONENTS, grads["steps"], g_r,
+                           np.zeros_like(g_r), aom=pop.aom, bc_inv=pop.bc_inv, cr=pop.cr, nthreads=threads)
+    cpu_sec = time.perf_counter() - t0
+    ref = np.concatenate([cr_, cv_], 1)
+    rel = np.max(np.abs(grads["got"] - ref), 1) / np.max(np.abs(ref), 1)
+    bc_rel = np.abs(grads["got_bc"] - cbc) / np.max(np.abs(cbc))
+    return {"grad_rel_err_max": float(rel.max()), "grad_rel_err_median": float(np.median(rel)),
             "bc_grad_rel_err_max": float(bc_rel.max()),
-            "cpu_reference": {"kind": "port", "method": "complex-step FP64 (7 complex propagations per object)",
-                              "gradient_steps_per_s": n * steps / cpu_sec, "cores": threads}}
+            "cpu_reference": {"kind": "port", "method": "oracle complex-step FP64 gradient (7 complex propagations "
+                                                        "per object)",
+                              "gradient_steps_per_s": pop.n * grads["steps"] / cpu_sec, "cores": threads}}
+
+
+def _pair_hash(a, b):
+    """Order-independent per-pair hash (splitmix64 finaliser of the id pair), for set digests."""
+    def mix(x):
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+    with np.errstate(over="ignore"):
+        return mix(a ^ mix(b))
+
+
+def ensemble_leg(la, steps, warmup=10):
+    """Batched ensemble: 256 scenarios of log-uniform 1,024-65,536 objects (~4M in total), each
+    a seeded population of the 16,384-object distributions; full force model, split
+    precision, dt = 10 s, detection every step (10 km cutoff, 1 km threshold), burn-up below
+    150 km.  Each step is one replay of a captured HIP graph with per-scenario cell grids."""
+    rng = np.random.default_rng(0x1ADD5 + 3)
+    sizes = np.exp(rng.uniform(np.log(1024), np.log(65536), 256)).astype(np.int64)
+    pop, off = la.generate_ensemble(sizes, (0x1ADD5 + 3) << 8, workload=0)
+    out = {"workload": f"{len(sizes)} scenarios, {pop.n} objects (log-uniform 1024-65536 per scenario), full "
+                       f"force model, split precision, dt 10 s, {steps} steps, detection every step",
+           "objects": int(pop.n), "steps": steps}
+    res = {}
+    cfg = la.EngineConfig(dt=10.0, components=la.ALL_COMPONENTS, precision=la.SPLIT, cutoff_km=10.0,
+                          threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, log_capacity=1 << 22)
+    with la.Engine(cfg) as eng:
+        eng.set_objects(pop)
+        eng.set_scenarios(off)
+        tuning = eng.autotune(repetitions=2)
+        eng.simulate(warmup, tpcd=1, min_altitude_km=MIN_ALT)
+        for graph in (1, 0):  # the same tuned context, steps replayed from a graph or launched eagerly
+            eng.configure(step_graph=graph)
+            eng.clear_log()
+            eng.set_objects(pop)
+            eng.set_scenarios(off)
+            eng.set_time(0.0)
+            eng.synchronize()
+            t0 = time.perf_counter()
+            st = eng.simulate(steps, tpcd=1, min_altitude_km=MIN_ALT)
+            eng.synchronize()
+            el = time.perf_counter() - t0
+            res[graph] = (el, st, eng.fetch_log(), tuning)
+    el, st, log, tuning = res[1]
+    scen = log["scenario"].astype(np.int64)
+    h = _pair_hash(log["pair"]["id_a"], log["pair"]["id_b"] ^ (log["iteration"] << np.uint64(40)))
+    counts = np.bincount(scen, minlength=len(sizes))
+    digests = np.zeros(len(sizes), np.uint64)
+    with np.errstate(over="ignore"):
+        np.add.at(digests, scen, h)
+    el0, st0, log0, _ = res[0]
+    same = (st["collisions"] == st0["collisions"] and np.array_equal(np.sort(h), np.sort(
+        _pair_hash(log0["pair"]["id_a"], log0["pair"]["id_b"] ^ (log0["iteration"] << np.uint64(40))))))
+    out.update({"object_steps_per_s": pop.n * steps / el, "ms_per_step": 1e3 * el / steps,
+                "eager_object_steps_per_s": pop.n * steps / el0, "graph_equals_eager": bool(same),
+                "collisions": int(st["collisions"]), "evasions": int(st["evasions"]),
+                "remaining": int(st["remaining"]), "tuning": tuning,
+                "pairs_per_scenario": counts.tolist(),
+                "pairset_digest_per_scenario": [f"{int(d):016x}" for d in digests]})
+    return out
+
+
+def clustered_leg(la, steps):
+    """8,388,608 objects, 60 % replicated constellation slots (+-50 m) and 40 % background;
+    split precision, dt = 10 s, cutoff 5 km, 0.5 km threshold, pair output as count + digest;
+    the autotuner picks cell size and tile shape.  The FP64 and split forms' digests of the
+    initial state are compared.  Timed over `steps` steps of the 1,000-step configuration."""
+    n = 8388608
+    pop = la.generate_population(2, 0x1ADD5 + 2, n)
+    out = {"workload": f"{n} objects clustered (constellation shells), split precision, dt 10 s, cutoff 5 km, "
+                       f"threshold 0.5 km, digest output; {steps} of 1000 steps timed", "objects": n, "steps": steps}
+    digests = {}
+    for prec, name in ((la.FP64, "fp64"), (la.SPLIT, "split")):
+        cfg = la.EngineConfig(dt=10.0, components=la.ALL_COMPONENTS, precision=prec, cutoff_km=5.0,
+                              threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=0.5, pair_output=la.PAIRS_DIGEST,
+                              log_capacity=1 << 16)
+        with la.Engine(cfg) as eng:
+            eng.set_objects(pop)
+            tuning = eng.autotune(repetitions=1)
+            digests[name] = eng.conjunction_digest(10.0)
+            if name == "split":
+                eng.synchronize()
+                t0 = time.perf_counter()
+                st = eng.simulate(steps, tpcd=1, min_altitude_km=MIN_ALT)
+                eng.synchronize()
+                el = time.perf_counter() - t0
+                out.update({"object_steps_per_s": n * steps / el, "ms_per_step": 1e3 * el / steps,
+                            "conjunctions_per_step": st["collisions"] / max(steps, 1), "tuning": tuning})
+    out["initial_conjunctions"] = int(digests["split"][0])
+    out["split_digest_equals_fp64"] = bool(digests["split"] == digests["fp64"])
+    return out
 
 
 def main():
@@ -268,11 +373,22 @@ def main():
                            "avg_us": kern[dom]["avg_us"], "bytes_per_object": per_obj}
     out["tuning"] = {form: res[form]["tuning"] for form in res}
     out["kernels_us"] = {form: {k: round(v["avg_us"], 2) for k, v in res[form]["kernels"].items()} for form in res}
+    grads = None
     if not a.no_adjoint:
-        out["adjoint"] = adjoint_leg(la)
+        out["adjoint"], grads = adjoint_leg(la)
         print(f"# adjoint: {json.dumps(out['adjoint'])}", file=sys.stderr, flush=True)
+    if not a.no_ensemble:
+        out["ensemble"] = ensemble_leg(la, a.ensemble_steps)
+        e = out["ensemble"]
+        print(f"# ensemble: {e['object_steps_per_s']:.4g} object-steps/s (eager {e['eager_object_steps_per_s']:.4g}),"
+              f" graph==eager {e['graph_equals_eager']}, collisions {e['collisions']}", file=sys.stderr, flush=True)
+    if not a.no_clustered:
+        out["clustered"] = clustered_leg(la, a.clustered_steps)
+        print(f"# clustered: {json.dumps(out['clustered'])}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pop, a.cpu_steps)
+        if grads is not None:
+            out["adjoint"].update(cpu_baseline_adjoint(la, grads))
     print(json.dumps(out))
 
 

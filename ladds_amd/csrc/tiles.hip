@@ -203,7 +203,7 @@ __device__ __forceinline__ uint32_t bucket_begin(const uint32_t *cur, uint32_t b
 // the buckets fz-1, fz, fz+1 (mod Kz) are one contiguous run of the sorted array, or two
 // runs where the fold wraps.  The rows are walked by one loop so that the candidate path
 // (the exact predicate and the wavefront compaction) exists once in the code.
-template <int FORM, bool DIGEST>
+template <int FORM, bool DIGEST, bool CHUNKED>
 __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FORM> *own, const RecOf<FORM> *halo,
                                            const uint32_t *cur_own, const uint32_t *cur_halo, uint32_t n_own,
                                            uint32_t threads, uint32_t scen, uint32_t chunk, uint32_t nchunk,
@@ -216,7 +216,7 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
     const RecOf<FORM> me = own[s];
     // chunks split the plane's objects by object index: the placement order differs
     // between the chunks' own sorts, the index does not
-    if (nchunk > 1u && me.idx % nchunk != chunk) continue;
+    if (CHUNKED && me.idx % nchunk != chunk) continue;
     double x, y, z;
     binning_point_of(me, x, y, z);
     const uint32_t lb = cell_of(x, y, z, g).bucket;
@@ -230,7 +230,7 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
       // sorted position.  The in-LDS placement order within a bucket depends on atomic
       // timing, so chunks of one plane (each sorting the plane itself) cannot share that
       // rule: they take a pair from its lower object index instead.
-      const bool by_index = !next_plane && nchunk > 1u;
+      const bool by_index = CHUNKED && !next_plane;
       const uint32_t from = (next_plane || by_index) ? 0u : s + 1u;
       const uint32_t row = ((fy + (kind >> 1) - 1u) & my) << sy;
       uint32_t lo0, hi0, lo1 = 0u, hi1 = 0u;
@@ -254,8 +254,8 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
 #pragma unroll 1
         for (; q < end; ++q) {
           const RecOf<FORM> other = arr[q];
-          if ((!by_index || other.idx > me.idx) && within<FORM>(me, other, a.sink.fp))
-            handle_candidate<DIGEST>(a.sink, me.idx, other.idx, acc, scen);
+          if (CHUNKED && by_index && other.idx <= me.idx) continue;
+          if (within<FORM>(me, other, a.sink.fp)) handle_candidate<DIGEST>(a.sink, me.idx, other.idx, acc, scen);
         }
         q = max(lo1, from);
         end = hi1;
@@ -419,11 +419,19 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
       }
     }
     // cur[b] now holds the end of bucket b
-    if (in_lds)
-      scan_plane<FORM, DIGEST>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, THREADS, scen, chunk,
-                               cp, acc);
-    else
-      scan_plane<FORM, DIGEST>(a, own, halo, cur_own, cur_next, n_own, THREADS, scen, chunk, cp, acc);
+    if (cp > 1u) {
+      if (in_lds)
+        scan_plane<FORM, DIGEST, true>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, THREADS, scen,
+                                       chunk, cp, acc);
+      else
+        scan_plane<FORM, DIGEST, true>(a, own, halo, cur_own, cur_next, n_own, THREADS, scen, chunk, cp, acc);
+    } else {
+      if (in_lds)
+        scan_plane<FORM, DIGEST, false>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, THREADS, scen,
+                                        0u, 1u, acc);
+      else
+        scan_plane<FORM, DIGEST, false>(a, own, halo, cur_own, cur_next, n_own, THREADS, scen, 0u, 1u, acc);
+    }
   }
   flush_counters<DIGEST>(a.sink, acc);
 }
