@@ -222,8 +222,9 @@ LADDS_API int ladds_delete_burnups(ladds_ctx *ctx, double min_altitude_km, size_
  * iteratePairwise calls of Simulation.cpp:211-222, whose return value reports
  * "still tuning"): time the plane-tile conjunction scan on the current state for
  * candidate workgroups per plane (1-32, for planes dense with clusters), pair-kernel
- * workgroup sizes, cell sizes (1, 1.25, 1.5, 2 x the search radius), plane shapes (log2
- * buckets per x-plane around the population-derived default) and partition tile sizes
+ * workgroup sizes, cell sizes (1, 1.25, 1.5, 2 x the search radius), plane widths (1, 1.5,
+ * 2, 3 x the cell size), plane shapes (log2 buckets per x-plane around the
+ * population-derived default) and partition tile sizes
  * (4096, 8192, 16384 objects), one dimension at a time (coordinate descent), and keep the
  * fastest for this context.  Pair results do not depend on the choice. */
 typedef struct ladds_tuning {
@@ -236,6 +237,7 @@ typedef struct ladds_tuning {
   int32_t tile_size;     /* chosen objects per partition tile                   */
   int32_t chunks;        /* chosen workgroups per plane (dense planes)          */
   int32_t reserved;
+  double plane_width_km; /* chosen cell length along x (plane width) [km]       */
 } ladds_tuning;
 
 LADDS_API int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuning *out);

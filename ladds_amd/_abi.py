@@ -108,6 +108,7 @@ class Tuning(C.Structure):
         ("tile_size", C.c_int32),
         ("chunks", C.c_int32),
         ("reserved", C.c_int32),
+        ("plane_width_km", C.c_double),
     ]
 
 

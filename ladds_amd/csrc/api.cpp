@@ -107,6 +107,7 @@ struct ladds_ctx {
   int tune_threads = 0;
   int tune_tile = 0;
   int tune_chunks = 0;
+  double tune_fx = 0.0;  // plane width / cell size
 
   StepState *d_step = nullptr;  // graph replay position (ladds_config.step_graph)
 
@@ -236,6 +237,9 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
                                             : "cell_size_km must be >= 2 * the search radius");
   GridParams g{};
   g.inv_cell = 1.0 / cell;
+  // plane width: cells may be longer in x than in y and z (fewer, fuller planes)
+  const double cell_x = (tiles && ctx->tune_fx > 1.0) ? cell * ctx->tune_fx : cell;
+  g.inv_cell_x = 1.0 / cell_x;
   uint32_t bits;
   std::vector<uint32_t> plane_count;
   if (tiles) {
@@ -244,7 +248,7 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
       xmax = 0.0;
       for (size_t i = 0; i < n; ++i) xmax = std::max(xmax, std::fabs(rx[i]));
     }
-    const double nx = std::min(1.0e6, 2.0 * xmax / cell + 3.0);
+    const double nx = std::min(1.0e6, 2.0 * xmax / cell_x + 3.0);
     g.bx = std::min<uint32_t>(14u, std::max<uint32_t>(2u, ceil_log2((uint64_t)nx)));
     // a scenario of an ensemble is a population of its own: its planes hold ~64 objects
     const size_t nscen = ctx->scen_off.size() > 2 ? ctx->scen_off.size() - 1 : 1;
@@ -263,7 +267,7 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
       for (size_t sc = 0; sc < nscen; ++sc) {
         const size_t lo = nscen > 1 ? ctx->scen_off[sc] : 0, hi = nscen > 1 ? ctx->scen_off[sc + 1] : n;
         for (size_t i = lo; i < hi; ++i) {
-          double u = rx[i] * g.inv_cell;
+          double u = rx[i] * g.inv_cell_x;
           u = std::fmin(std::fmax(u, -1.0e9), 1.0e9);
           plane_count[sc * P + ((uint32_t)(int32_t)std::floor(u) & mx)] += 1;
         }
@@ -891,20 +895,22 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   ctx->tune_threads = 0;
   ctx->tune_tile = 0;
   ctx->tune_chunks = 0;
+  ctx->tune_fx = 0.0;
   int rc = setup_grid(ctx, n, rx.data());
   if (rc != LADDS_OK) return rc;
   const double base_cell = 1.0 / ctx->grid.inv_cell;
   const int base_pb = (int)(ctx->grid.by + ctx->grid.bz);
   // current point of the search: {cell, plane bits, threads, tile, chunks}
-  double cur_cell = base_cell;
+  double cur_cell = base_cell, cur_fx = 1.0;
   int cur[4] = {base_pb, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk};
   double default_us = 0.0;
   rc = time_detection(ctx, dt_interp, repetitions, &default_us);
   if (rc != LADDS_OK) return rc;
   double best_us = default_us;
   int tried = 1;
-  auto apply = [&](double cell, const int *v) {
+  auto apply = [&](double cell, double fx, const int *v) {
     ctx->tune_cell_km = cell;
+    ctx->tune_fx = fx;
     ctx->tune_pb = v[0];
     ctx->tune_threads = v[1];
     ctx->tune_tile = v[2];
@@ -917,25 +923,32 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   const int thread_opts[] = {128, 256, 512, 1024};
   const double cell_opts[] = {1.0, 1.25, 1.5, 2.0};
   const int tile_opts[] = {4096, 8192, 16384};
-  for (int dim = 0; dim < 5; ++dim) {
-    std::vector<std::pair<double, std::array<int, 4>>> cands;
+  const double fx_opts[] = {1.0, 1.5, 2.0, 3.0};
+  struct Cand {
+    double cell, fx;
+    std::array<int, 4> v;
+  };
+  for (int dim = 0; dim < 6; ++dim) {
+    std::vector<Cand> cands;
     std::array<int, 4> c{cur[0], cur[1], cur[2], cur[3]};
     if (dim == 0)
-      for (int k : chunk_opts) { c[3] = k; cands.push_back({cur_cell, c}); }
+      for (int k : chunk_opts) { c[3] = k; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 1)
-      for (int t : thread_opts) { c[1] = t; cands.push_back({cur_cell, c}); }
+      for (int t : thread_opts) { c[1] = t; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 2)
-      for (double f : cell_opts) cands.push_back({base_cell * f, c});
+      for (double f : cell_opts) cands.push_back({base_cell * f, cur_fx, c});
     if (dim == 3)
-      for (int d = -1; d <= 2; ++d)
-        if (base_pb + d >= 4 && base_pb + d <= 12) { c[0] = base_pb + d; cands.push_back({cur_cell, c}); }
+      for (double f : fx_opts) cands.push_back({cur_cell, f, c});
     if (dim == 4)
-      for (int t : tile_opts) { c[2] = t; cands.push_back({cur_cell, c}); }
-    for (const auto &cand : cands) {
-      if (cand.first == cur_cell && cand.second[0] == cur[0] && cand.second[1] == cur[1] &&
-          cand.second[2] == cur[2] && cand.second[3] == cur[3])
+      for (int d = -1; d <= 2; ++d)
+        if (base_pb + d >= 4 && base_pb + d <= 12) { c[0] = base_pb + d; cands.push_back({cur_cell, cur_fx, c}); }
+    if (dim == 5)
+      for (int t : tile_opts) { c[2] = t; cands.push_back({cur_cell, cur_fx, c}); }
+    for (const Cand &cand : cands) {
+      if (cand.cell == cur_cell && cand.fx == cur_fx && cand.v[0] == cur[0] && cand.v[1] == cur[1] &&
+          cand.v[2] == cur[2] && cand.v[3] == cur[3])
         continue;
-      rc = apply(cand.first, cand.second.data());
+      rc = apply(cand.cell, cand.fx, cand.v.data());
       if (rc != LADDS_OK) return rc;
       double us = 0.0;
       rc = time_detection(ctx, dt_interp, repetitions, &us);
@@ -943,18 +956,20 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
       ++tried;
       if (us < best_us) {
         best_us = us;
-        cur_cell = cand.first;
-        for (int k = 0; k < 4; ++k) cur[k] = cand.second[k];
+        cur_cell = cand.cell;
+        cur_fx = cand.fx;
+        for (int k = 0; k < 4; ++k) cur[k] = cand.v[k];
       }
     }
   }
-  rc = apply(cur_cell, cur);
+  rc = apply(cur_cell, cur_fx, cur);
   if (rc != LADDS_OK) return rc;
   HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (out) {
     out->cell_size_km = 1.0 / ctx->grid.inv_cell;
+    out->plane_width_km = 1.0 / ctx->grid.inv_cell_x;
     out->plane_bits = (int32_t)(ctx->grid.by + ctx->grid.bz);
     out->pair_threads = (int32_t)ctx->tiles.pair_threads;
     out->tile_size = (int32_t)ctx->tiles.tile_size;

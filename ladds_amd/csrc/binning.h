@@ -52,7 +52,7 @@ __device__ __forceinline__ int32_t cell_axis(double x, double inv_cell, uint32_t
 
 __device__ __forceinline__ CellInfo cell_of(double x, double y, double z, const GridParams &g) {
   uint32_t lx, ly, lz;
-  const int32_t ix = cell_axis(x, g.inv_cell, lx);
+  const int32_t ix = cell_axis(x, g.inv_cell_x, lx);
   const int32_t iy = cell_axis(y, g.inv_cell, ly);
   const int32_t iz = cell_axis(z, g.inv_cell, lz);
   const uint32_t mx = (1u << g.bx) - 1u, my = (1u << g.by) - 1u, mz = (1u << g.bz) - 1u;

@@ -37,7 +37,8 @@ static_assert(sizeof(SubstepEnv<float>) == 20 * 4, "env layout");
 // spatially coherent runs; cells that alias into one bucket are at least
 // K * cell_size apart and are rejected by the distance test.
 struct GridParams {
-  double inv_cell;       // 1 / cell size [1/km]
+  double inv_cell;       // 1 / cell size in y and z [1/km]
+  double inv_cell_x;     // 1 / cell size in x (the plane width) [1/km], >= cell size
   uint32_t bx, by, bz;   // log2 of Kx, Ky, Kz
   uint32_t nbuckets;     // Kx * Ky * Kz
 };
