@@ -111,6 +111,22 @@ struct ladds_ctx {
 
   StepState *d_step = nullptr;  // graph replay position (ladds_config.step_graph)
 
+  // Overlapped simulation: step k+1's propagation and partition run on `stream` while
+  // step k's pair scan runs on `stream2`.  The state (r, v, activity), the records, the
+  // tile offsets and the scratch arena are double-buffered; buffer 0 is `d` itself.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_part[2] = {nullptr, nullptr}, ev_pair[2] = {nullptr, nullptr};
+  double *sb_r[3] = {nullptr, nullptr, nullptr}, *sb_v[3] = {nullptr, nullptr, nullptr};
+  int32_t *sb_act = nullptr;
+  size_t sb_cap = 0;
+  void *rec2 = nullptr;
+  size_t rec2_cap = 0;
+  uint32_t *tps2 = nullptr;
+  size_t tps2_alloc = 0;
+  uint32_t *arena_top2 = nullptr;
+  void *scratch2 = nullptr;
+  size_t scratch2_bytes = 0;
+
   uint64_t step_count = 0;  // steps since t0
   double c[4], dcoef[3];
 
@@ -151,20 +167,26 @@ namespace {
 
 // Launch wrapper: records events around the launch when profiling is on.
 template <typename F>
-hipError_t timed(ladds_ctx *ctx, const char *name, F &&launch) {
+hipError_t timed_on(ladds_ctx *ctx, const char *name, hipStream_t stream, F &&launch) {
   if (!ctx->profiling) return launch();
   EventPair ev = ctx->get_events();
-  (void)hipEventRecord(ev.a, ctx->stream);
+  (void)hipEventRecord(ev.a, stream);
   const hipError_t e = launch();
-  (void)hipEventRecord(ev.b, ctx->stream);
+  (void)hipEventRecord(ev.b, stream);
   KernelProfile &kp = ctx->prof[name];
   kp.used.push_back(ev);
   kp.launches += 1;
   return e;
 }
 
+template <typename F>
+hipError_t timed(ladds_ctx *ctx, const char *name, F &&launch) {
+  return timed_on(ctx, name, ctx->stream, launch);
+}
+
 void resolve_profiles(ladds_ctx *ctx) {
   hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
   for (auto &kv : ctx->prof) {
     for (EventPair &ev : kv.second.used) {
       float ms = 0.f;
@@ -497,6 +519,102 @@ int launch_detection(ladds_ctx *ctx, bool binned, double dt_interp, bool digest,
   return LADDS_OK;
 }
 
+// Second state buffer and second records / offsets / arena for the overlapped loop.
+int ensure_overlap_buffers(ladds_ctx *ctx) {
+  if (ctx->sb_cap < ctx->cap) {
+    for (int k = 0; k < 3; ++k) {
+      hipFree(ctx->sb_r[k]);
+      hipFree(ctx->sb_v[k]);
+      ctx->sb_r[k] = ctx->sb_v[k] = nullptr;
+    }
+    hipFree(ctx->sb_act);
+    ctx->sb_act = nullptr;
+    ctx->sb_cap = 0;
+    for (int k = 0; k < 3; ++k) {
+      HIPCHK(ctx, dalloc(&ctx->sb_r[k], ctx->cap));
+      HIPCHK(ctx, dalloc(&ctx->sb_v[k], ctx->cap));
+    }
+    HIPCHK(ctx, dalloc(&ctx->sb_act, ctx->cap));
+    ctx->sb_cap = ctx->cap;
+  }
+  if (ctx->rec2_cap < ctx->cap) {
+    hipFree(ctx->rec2);
+    ctx->rec2 = nullptr;
+    ctx->rec2_cap = 0;
+    HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&ctx->rec2), ctx->cap));
+    ctx->rec2_cap = ctx->cap;
+  }
+  if (ctx->tps2_alloc < ctx->tps_alloc) {
+    hipFree(ctx->tps2);
+    ctx->tps2 = nullptr;
+    ctx->tps2_alloc = 0;
+    HIPCHK(ctx, dalloc(&ctx->tps2, ctx->tps_alloc));
+    ctx->tps2_alloc = ctx->tps_alloc;
+  }
+  if (ctx->scratch2_bytes < ctx->scratch_bytes) {
+    hipFree(ctx->scratch2);
+    ctx->scratch2 = nullptr;
+    ctx->scratch2_bytes = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->scratch2, ctx->scratch_bytes));
+    ctx->scratch2_bytes = ctx->scratch_bytes;
+  }
+  return LADDS_OK;
+}
+
+// View of state buffer b (0: the context's own arrays).
+DeviceObjects state_view(const ladds_ctx *ctx, int b) {
+  DeviceObjects v = ctx->d;
+  if (b) {
+    v.rx = ctx->sb_r[0];
+    v.ry = ctx->sb_r[1];
+    v.rz = ctx->sb_r[2];
+    v.vx = ctx->sb_v[0];
+    v.vy = ctx->sb_v[1];
+    v.vz = ctx->sb_v[2];
+    v.activity = ctx->sb_act;
+  }
+  return v;
+}
+
+// One overlapped iteration: propagate from buffer `cur` into buffer 1 - cur (stream),
+// and when `detect`, partition it (stream) and scan its pairs (stream2).
+int overlapped_iteration(ladds_ctx *ctx, int cur, bool pending_next, size_t env_offset, int nsteps, bool detect,
+                         double burnup_r2, double dt_interp, bool digest, uint64_t iteration) {
+  const int nxt = 1 - cur;
+  // buffer nxt was last read by a pair scan on stream2
+  if (pending_next) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_pair[nxt], 0));
+  const DeviceObjects in = state_view(ctx, cur), out = state_view(ctx, nxt);
+  double cdt[4], ddt[3];
+  for (int k = 0; k < 4; ++k) cdt[k] = ctx->c[k] * ctx->cfg.dt;
+  for (int k = 0; k < 3; ++k) ddt[k] = ctx->dcoef[k] * ctx->cfg.dt;
+  hipError_t e = timed(ctx, "propagate", [&] {
+    return launch_propagate(ctx->form, in, ctx->env64 + 3 * env_offset, ctx->env32 + 3 * env_offset, nsteps,
+                            ctx->cfg.components, cdt, ddt, &ctx->grid, detect ? ctx->slot : nullptr, nullptr,
+                            ctx->slot, burnup_r2, nullptr, ctx->stream, &out);
+  });
+  if (e != hipSuccess) return ctx->hipfail(e, "propagate launch");
+  if (!detect) return LADDS_OK;
+  void *rec = nxt ? ctx->rec2 : ctx->rec;
+  uint32_t *tps = nxt ? ctx->tps2 : ctx->tps;
+  uint32_t *arena_top = nxt ? ctx->arena_top2 : ctx->arena_top;
+  void *scratch = nxt ? ctx->scratch2 : ctx->scratch;
+  e = timed(ctx, "tile_partition", [&] {
+    return launch_tile_partition(ctx->form, out, ctx->slot, ctx->tiles, rec, tps, arena_top, ctx->stream);
+  });
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev_part[nxt], ctx->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream2, ctx->ev_part[nxt], 0);
+  const FunctorParams fp = functor_params(ctx, dt_interp);
+  if (e == hipSuccess)
+    e = timed_on(ctx, "pair_tiles", ctx->stream2, [&] {
+      return launch_pair_tiles(ctx->form, digest, out, ctx->grid, ctx->tiles, fp, rec, tps, scratch, arena_top,
+                               ctx->log, ctx->log_cap, ctx->log_count, ctx->counters, iteration, nullptr,
+                               ctx->stream2);
+    });
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev_pair[nxt], ctx->stream2);
+  if (e != hipSuccess) return ctx->hipfail(e, "overlapped detection launch");
+  return LADDS_OK;
+}
+
 int compact_if_needed(ladds_ctx *ctx) {
   if (ctx->n == 0) return LADDS_OK;
   const uint32_t ntiles = (uint32_t)((ctx->n + kCompactTile - 1) / kCompactTile);
@@ -596,6 +714,12 @@ int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
   yoshida_coefficients(ctx->c, ctx->dcoef);
   hipError_t e = hipSetDevice(ctx->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
+  for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+    e = hipEventCreateWithFlags(&ctx->ev_part[b], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_pair[b], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = dalloc(&ctx->arena_top2, 1);
   if (e == hipSuccess) e = upload_atmosphere();
   if (e == hipSuccess) e = configure_tile_kernels();
   if (e == hipSuccess) e = dalloc(&ctx->arena_top, 1);
@@ -621,6 +745,21 @@ int ladds_destroy(ladds_ctx *ctx) {
   if (!ctx) return LADDS_E_INVALID;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
+  for (int k = 0; k < 3; ++k) {
+    hipFree(ctx->sb_r[k]);
+    hipFree(ctx->sb_v[k]);
+  }
+  hipFree(ctx->sb_act);
+  hipFree(ctx->rec2);
+  hipFree(ctx->tps2);
+  hipFree(ctx->arena_top2);
+  hipFree(ctx->scratch2);
+  for (int b = 0; b < 2; ++b) {
+    if (ctx->ev_part[b]) hipEventDestroy(ctx->ev_part[b]);
+    if (ctx->ev_pair[b]) hipEventDestroy(ctx->ev_pair[b]);
+  }
+  if (ctx->stream2) hipStreamDestroy(ctx->stream2);
   free_objects(ctx->d);
   free_objects(ctx->d_alt);
   hipFree(ctx->count);
@@ -1085,6 +1224,15 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     if (graph) hipGraphDestroy(graph);
     if (e != hipSuccess) return ctx->hipfail(e, "step graph capture");
   }
+  // Overlap of propagation + partition (stream) with the pair scan (stream2): the plane-tile
+  // scan launched eagerly.
+  const bool use_overlap = !use_graph && ctx->variant == kVariantPlaneTiles && ctx->n > 0 && nsteps > 0;
+  int cur = 0;                       // state buffer holding the latest state
+  bool pending[2] = {false, false};  // a pair scan reading buffer b may still run on stream2
+  if (use_overlap) {
+    const int rc = ensure_overlap_buffers(ctx);
+    if (rc != LADDS_OK) return rc;
+  }
   for (int seg0 = 0; seg0 < nsteps && ctx->n > 0; seg0 += kEnvSegment) {
     const int seg = std::min(kEnvSegment, nsteps - seg0);
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // pinned environment buffer is free again
@@ -1124,12 +1272,35 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
         ++it;
       }
       const bool detect = (it % (uint64_t)tpcd) == 0;
-      int rc = launch_propagate_steps(ctx, (size_t)s, k, detect, burnup_r2);
-      if (rc == LADDS_OK && detect) rc = launch_detection(ctx, true, ctx->cfg.dt * (double)tpcd, digest, it);
+      int rc;
+      if (use_overlap) {
+        rc = overlapped_iteration(ctx, cur, pending[1 - cur], (size_t)s, k, detect, burnup_r2,
+                                  ctx->cfg.dt * (double)tpcd, digest, it);
+        if (rc == LADDS_OK) {
+          if (detect) pending[1 - cur] = true;
+          cur = 1 - cur;
+        }
+      } else {
+        rc = launch_propagate_steps(ctx, (size_t)s, k, detect, burnup_r2);
+        if (rc == LADDS_OK && detect) rc = launch_detection(ctx, true, ctx->cfg.dt * (double)tpcd, digest, it);
+      }
       if (rc != LADDS_OK) return rc;
       s += k;
     }
     ctx->step_count += (uint64_t)seg;
+  }
+  if (use_overlap) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream2));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (cur == 1) {  // the final state is in the second buffer: make it the context's own
+      std::swap(ctx->d.rx, ctx->sb_r[0]);
+      std::swap(ctx->d.ry, ctx->sb_r[1]);
+      std::swap(ctx->d.rz, ctx->sb_r[2]);
+      std::swap(ctx->d.vx, ctx->sb_v[0]);
+      std::swap(ctx->d.vy, ctx->sb_v[1]);
+      std::swap(ctx->d.vz, ctx->sb_v[2]);
+      std::swap(ctx->d.activity, ctx->sb_act);
+    }
   }
   ScanCounters sc{};
   unsigned long long cnt = 0;

@@ -36,6 +36,9 @@ struct PropagateArgs {
   uint32_t *slot;
   double burnup_r2;  // > 0: objects with |r|^2 below become dummies (deleteBurnUps)
   const StepState *step_state;  // non-null (graph replay): environments start at step_state->step
+  // output state (the input arrays of `o` when propagating in place)
+  double *orx, *ory, *orz, *ovx, *ovy, *ovz;
+  int32_t *oact;
 };
 
 template <int FORM>
@@ -88,13 +91,16 @@ __global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
       break;
     }
   }
-  if (burnt) a.o.activity[i] = -1;
-  a.o.rx[i] = x;
-  a.o.ry[i] = y;
-  a.o.rz[i] = z;
-  a.o.vx[i] = vx;
-  a.o.vy[i] = vy;
-  a.o.vz[i] = vz;
+  if (burnt)
+    a.oact[i] = -1;
+  else if (a.oact != a.o.activity)
+    a.oact[i] = p.activity;
+  a.orx[i] = x;
+  a.ory[i] = y;
+  a.orz[i] = z;
+  a.ovx[i] = vx;
+  a.ovy[i] = vy;
+  a.ovz[i] = vz;
   if (a.keys || a.count) {
     binning_point<FORM != kFormFp64>(x, y, z);
     const CellInfo c = cell_of(x, y, z, a.grid);
@@ -114,10 +120,19 @@ hipError_t upload_atmosphere() {
 hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<double> *env64,
                             const SubstepEnv<float> *env32, int nsteps, unsigned mask, const double cdt[4],
                             const double ddt[3], const GridParams *grid, uint32_t *keys, uint32_t *count,
-                            uint32_t *slot, double burnup_r2, const StepState *step_state, hipStream_t stream) {
+                            uint32_t *slot, double burnup_r2, const StepState *step_state, hipStream_t stream,
+                            const DeviceObjects *out) {
   if (o.n == 0 || nsteps <= 0) return hipSuccess;
   PropagateArgs a;
   a.o = o;
+  const DeviceObjects &w = out ? *out : o;
+  a.orx = w.rx;
+  a.ory = w.ry;
+  a.orz = w.rz;
+  a.ovx = w.vx;
+  a.ovy = w.vy;
+  a.ovz = w.vz;
+  a.oact = w.activity;
   a.env64 = env64;
   a.env32 = env32;
   a.nsteps = nsteps;
