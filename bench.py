@@ -72,6 +72,9 @@ def parse():
                    help="timed steps of the clustered leg (each ~0.4 s: the workload has ~2.7e9 conjunctions "
                         "per step)")
     p.add_argument("--forms", default="fp64,split")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep the default plane-tile shape (profiler runs: the tuner's timings change under "
+                        "counter collection)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="first copy MIB MiB device-to-device 3 times (a plain copy kernel of known size, "
                         "for scaling hardware byte counters), then run the bench")
@@ -95,11 +98,11 @@ def make_engine(la, precision):
     return la.Engine(cfg)
 
 
-def run_form(la, pop, precision, steps, warmup, profile_steps):
+def run_form(la, pop, precision, steps, warmup, profile_steps, autotune=True):
     eng = make_engine(la, precision)
     eng.set_objects(pop)
     # plane-tile shape picked by timing the scan on the initial state (outside the timed region)
-    tuning = eng.autotune(repetitions=3)
+    tuning = eng.autotune(repetitions=3) if autotune else None
     if warmup:
         eng.simulate(warmup, first_iteration=0, tpcd=1, min_altitude_km=MIN_ALT)
     eng.clear_log()
@@ -324,7 +327,7 @@ def main():
     forms = {"fp64": la.FP64, "split": la.SPLIT}
     res = {}
     for name in a.forms.split(","):
-        res[name] = run_form(la, pop, forms[name], a.steps, a.warmup, a.profile_steps)
+        res[name] = run_form(la, pop, forms[name], a.steps, a.warmup, a.profile_steps, not a.no_autotune)
         r = res[name]
         print(f"# {name}: {a.steps} steps in {r['elapsed']:.3f} s, {a.objects * a.steps / r['elapsed']:.4g} "
               f"object-steps/s, collisions {r['stats']['collisions']}, candidates {r['stats']['candidates']}, "

@@ -90,6 +90,11 @@ __device__ __forceinline__ void binning_point_of(const REC &r, double &x, double
 // ---------------------------------------------------------------------------
 // keys and partition
 // ---------------------------------------------------------------------------
+// LDS words of the partition kernel before its staging area (32-byte aligned).
+__host__ __device__ constexpr uint32_t partition_header_words(uint32_t nplanes) {
+  return (nplanes + kPartThreads / 64 + 1 + 7u) & ~7u;
+}
+
 template <bool F32>
 __global__ __launch_bounds__(256) void k_bin_keys(const double *rx, const double *ry, const double *rz, size_t n,
                                                   GridParams g, uint32_t *keys) {
@@ -100,7 +105,10 @@ __global__ __launch_bounds__(256) void k_bin_keys(const double *rx, const double
   keys[i] = cell_of(x, y, z, g).bucket;
 }
 
-template <typename REC, int kPartItems>
+// STAGE: records are placed in LDS first and the tile's region is then written with
+// contiguous stores (scattered 16/32-byte stores into the region cost ~3x the bytes in
+// partial-line write-backs).  Needs tile_size records of LDS.
+template <typename REC, int kPartItems, bool STAGE>
 __global__ __launch_bounds__(kPartThreads) void k_tile_partition(const double *rx, const double *ry,
                                                                   const double *rz, size_t n, const uint32_t *keys,
                                                                   TileParams tp, REC *items, uint32_t *tps,
@@ -108,6 +116,7 @@ __global__ __launch_bounds__(kPartThreads) void k_tile_partition(const double *r
   extern __shared__ uint32_t lds[];
   uint32_t *cnt = lds;                // nplanes
   uint32_t *wsum = lds + tp.nplanes;  // kPartThreads / 64 + 1
+  REC *staged = reinterpret_cast<REC *>(lds + partition_header_words(tp.nplanes));
   const uint32_t P = tp.nplanes, pb = tp.plane_bits;
   // objects [base, end) of this tile
   size_t base, end;
@@ -161,7 +170,14 @@ __global__ __launch_bounds__(kPartThreads) void k_tile_partition(const double *r
     r.z = z;
     r.idx = (uint32_t)i;
     if constexpr (!f32) r.pad = 0;
-    items[base + pos] = r;
+    if constexpr (STAGE)
+      staged[pos] = r;
+    else
+      items[base + pos] = r;
+  }
+  if constexpr (STAGE) {
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < total; k += kPartThreads) items[base + k] = staged[k];
   }
 }
 
@@ -449,24 +465,33 @@ hipError_t launch_bin_keys(int form, const DeviceObjects &o, const GridParams &g
   return hipGetLastError();
 }
 
+template <typename REC, int ITEMS>
+void launch_partition_items(const DeviceObjects &o, const uint32_t *keys, const TileParams &tp, REC *out,
+                            uint32_t *tps, uint32_t *arena_top, hipStream_t s) {
+  const dim3 grid(tp.ntiles), block(kPartThreads);
+  const size_t head = sizeof(uint32_t) * partition_header_words(tp.nplanes);
+  const size_t staged = head + sizeof(REC) * (size_t)ITEMS * kPartThreads;
+  if (staged <= kPairTileLdsBytes)
+    hipLaunchKernelGGL((k_tile_partition<REC, ITEMS, true>), grid, block, staged, s, o.rx, o.ry, o.rz, o.n, keys, tp,
+                       out, tps, arena_top);
+  else
+    hipLaunchKernelGGL((k_tile_partition<REC, ITEMS, false>), grid, block, head, s, o.rx, o.ry, o.rz, o.n, keys, tp,
+                       out, tps, arena_top);
+}
+
 template <typename REC>
 hipError_t launch_tile_partition_as(const DeviceObjects &o, const uint32_t *keys, const TileParams &tp, void *items,
                                     uint32_t *tps, uint32_t *arena_top, hipStream_t s) {
-  const size_t shm = sizeof(uint32_t) * (tp.nplanes + kPartThreads / 64 + 1);
-  const dim3 grid(tp.ntiles), block(kPartThreads);
   REC *out = static_cast<REC *>(items);
   switch (tp.tile_size) {
     case 4096:
-      hipLaunchKernelGGL((k_tile_partition<REC, 4>), grid, block, shm, s, o.rx, o.ry, o.rz, o.n, keys, tp, out, tps,
-                         arena_top);
+      launch_partition_items<REC, 4>(o, keys, tp, out, tps, arena_top, s);
       break;
     case 8192:
-      hipLaunchKernelGGL((k_tile_partition<REC, 8>), grid, block, shm, s, o.rx, o.ry, o.rz, o.n, keys, tp, out, tps,
-                         arena_top);
+      launch_partition_items<REC, 8>(o, keys, tp, out, tps, arena_top, s);
       break;
     case 16384:
-      hipLaunchKernelGGL((k_tile_partition<REC, 16>), grid, block, shm, s, o.rx, o.ry, o.rz, o.n, keys, tp, out, tps,
-                         arena_top);
+      launch_partition_items<REC, 16>(o, keys, tp, out, tps, arena_top, s);
       break;
     default:
       return hipErrorInvalidValue;
@@ -503,12 +528,15 @@ hipError_t configure_tile_kernels() {
   auto set = [&](const void *f) {
     if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
   };
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 4>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 8>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 16>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 4>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 8>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 16>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 4, true>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 8, false>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 16, false>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 4, false>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 4, true>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 8, true>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 4, false>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 8, false>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 16, false>));
   return e;
 }
 
