@@ -576,6 +576,43 @@ DeviceObjects state_view(const ladds_ctx *ctx, int b) {
   return v;
 }
 
+// Propagate from state buffer `cur` into 1 - cur and, when `partition`, partition the result
+// (both on `stream`).  With `step_state` the environments start at its step + step_offset.
+int launch_prop_part(ladds_ctx *ctx, int cur, size_t env_offset, int nsteps, bool partition, double burnup_r2,
+                     const StepState *step_state, uint32_t step_offset) {
+  const int nxt = 1 - cur;
+  const DeviceObjects in = state_view(ctx, cur), out = state_view(ctx, nxt);
+  double cdt[4], ddt[3];
+  for (int k = 0; k < 4; ++k) cdt[k] = ctx->c[k] * ctx->cfg.dt;
+  for (int k = 0; k < 3; ++k) ddt[k] = ctx->dcoef[k] * ctx->cfg.dt;
+  hipError_t e = timed(ctx, "propagate", [&] {
+    return launch_propagate(ctx->form, in, ctx->env64 + 3 * env_offset, ctx->env32 + 3 * env_offset, nsteps,
+                            ctx->cfg.components, cdt, ddt, &ctx->grid, partition ? ctx->slot : nullptr, nullptr,
+                            ctx->slot, burnup_r2, step_state, ctx->stream, &out, step_offset);
+  });
+  if (e == hipSuccess && partition)
+    e = timed(ctx, "tile_partition", [&] {
+      return launch_tile_partition(ctx->form, out, ctx->slot, ctx->tiles, nxt ? ctx->rec2 : ctx->rec,
+                                   nxt ? ctx->tps2 : ctx->tps, nxt ? ctx->arena_top2 : ctx->arena_top, ctx->stream);
+    });
+  if (e != hipSuccess) return ctx->hipfail(e, "propagation launch");
+  return LADDS_OK;
+}
+
+// Pair scan of state buffer b (records partitioned into buffer b) on `stream`.
+int launch_pairs_of(ladds_ctx *ctx, int b, double dt_interp, bool digest, uint64_t iteration,
+                    const StepState *step_state, hipStream_t stream) {
+  const FunctorParams fp = functor_params(ctx, dt_interp);
+  const hipError_t e = timed_on(ctx, "pair_tiles", stream, [&] {
+    return launch_pair_tiles(ctx->form, digest, state_view(ctx, b), ctx->grid, ctx->tiles, fp,
+                             b ? ctx->rec2 : ctx->rec, b ? ctx->tps2 : ctx->tps, b ? ctx->scratch2 : ctx->scratch,
+                             b ? ctx->arena_top2 : ctx->arena_top, ctx->log, ctx->log_cap, ctx->log_count,
+                             ctx->counters, iteration, step_state, stream);
+  });
+  if (e != hipSuccess) return ctx->hipfail(e, "pair scan launch");
+  return LADDS_OK;
+}
+
 // One overlapped iteration: propagate from buffer `cur` into buffer 1 - cur (stream),
 // and when `detect`, partition it (stream) and scan its pairs (stream2).
 int overlapped_iteration(ladds_ctx *ctx, int cur, bool pending_next, size_t env_offset, int nsteps, bool detect,
@@ -583,35 +620,38 @@ int overlapped_iteration(ladds_ctx *ctx, int cur, bool pending_next, size_t env_
   const int nxt = 1 - cur;
   // buffer nxt was last read by a pair scan on stream2
   if (pending_next) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_pair[nxt], 0));
-  const DeviceObjects in = state_view(ctx, cur), out = state_view(ctx, nxt);
-  double cdt[4], ddt[3];
-  for (int k = 0; k < 4; ++k) cdt[k] = ctx->c[k] * ctx->cfg.dt;
-  for (int k = 0; k < 3; ++k) ddt[k] = ctx->dcoef[k] * ctx->cfg.dt;
-  hipError_t e = timed(ctx, "propagate", [&] {
-    return launch_propagate(ctx->form, in, ctx->env64 + 3 * env_offset, ctx->env32 + 3 * env_offset, nsteps,
-                            ctx->cfg.components, cdt, ddt, &ctx->grid, detect ? ctx->slot : nullptr, nullptr,
-                            ctx->slot, burnup_r2, nullptr, ctx->stream, &out);
-  });
-  if (e != hipSuccess) return ctx->hipfail(e, "propagate launch");
-  if (!detect) return LADDS_OK;
-  void *rec = nxt ? ctx->rec2 : ctx->rec;
-  uint32_t *tps = nxt ? ctx->tps2 : ctx->tps;
-  uint32_t *arena_top = nxt ? ctx->arena_top2 : ctx->arena_top;
-  void *scratch = nxt ? ctx->scratch2 : ctx->scratch;
-  e = timed(ctx, "tile_partition", [&] {
-    return launch_tile_partition(ctx->form, out, ctx->slot, ctx->tiles, rec, tps, arena_top, ctx->stream);
-  });
-  if (e == hipSuccess) e = hipEventRecord(ctx->ev_part[nxt], ctx->stream);
-  if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream2, ctx->ev_part[nxt], 0);
-  const FunctorParams fp = functor_params(ctx, dt_interp);
-  if (e == hipSuccess)
-    e = timed_on(ctx, "pair_tiles", ctx->stream2, [&] {
-      return launch_pair_tiles(ctx->form, digest, out, ctx->grid, ctx->tiles, fp, rec, tps, scratch, arena_top,
-                               ctx->log, ctx->log_cap, ctx->log_count, ctx->counters, iteration, nullptr,
-                               ctx->stream2);
-    });
-  if (e == hipSuccess) e = hipEventRecord(ctx->ev_pair[nxt], ctx->stream2);
-  if (e != hipSuccess) return ctx->hipfail(e, "overlapped detection launch");
+  int rc = launch_prop_part(ctx, cur, env_offset, nsteps, detect, burnup_r2, nullptr, 0);
+  if (rc != LADDS_OK || !detect) return rc;
+  HIPCHK(ctx, hipEventRecord(ctx->ev_part[nxt], ctx->stream));
+  HIPCHK(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_part[nxt], 0));
+  rc = launch_pairs_of(ctx, nxt, dt_interp, digest, iteration, nullptr, ctx->stream2);
+  if (rc != LADDS_OK) return rc;
+  HIPCHK(ctx, hipEventRecord(ctx->ev_pair[nxt], ctx->stream2));
+  return LADDS_OK;
+}
+
+// Software-pipelined step graph for state buffer b: the pair scan of the state in b (on a
+// forked branch) runs beside the propagation of b into 1 - b and its partition (one step
+// ahead, environment step + 1); then the device step advances.  Two graphs, b = 0 and 1,
+// alternate.  Nothing is memset inside: the partition kernel resets its scratch arena.
+int capture_step_graph(ladds_ctx *ctx, int b, double burnup_r2, bool digest, hipGraphExec_t *exec) {
+  *exec = nullptr;
+  hipGraph_t graph = nullptr;
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = LADDS_OK;
+  hipError_t e = hipEventRecord(ctx->ev_part[0], ctx->stream);  // fork
+  if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream2, ctx->ev_part[0], 0);
+  if (e == hipSuccess) rc = launch_pairs_of(ctx, b, ctx->cfg.dt, digest, 0, ctx->d_step, ctx->stream2);
+  if (e == hipSuccess && rc == LADDS_OK) rc = launch_prop_part(ctx, b, 0, 1, true, burnup_r2, ctx->d_step, 1u);
+  if (e == hipSuccess && rc == LADDS_OK) e = hipEventRecord(ctx->ev_pair[0], ctx->stream2);  // join
+  if (e == hipSuccess && rc == LADDS_OK) e = hipStreamWaitEvent(ctx->stream, ctx->ev_pair[0], 0);
+  if (e == hipSuccess && rc == LADDS_OK) e = launch_advance_step(ctx->d_step, ctx->stream);
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc == LADDS_OK && e == hipSuccess) e = ec;
+  if (rc == LADDS_OK && e == hipSuccess) e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  if (graph) hipGraphDestroy(graph);
+  if (rc != LADDS_OK) return rc;
+  if (e != hipSuccess) return ctx->hipfail(e, "step graph capture");
   return LADDS_OK;
 }
 
@@ -1199,58 +1239,70 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     const int rc = alloc_env(ctx);
     if (rc != LADDS_OK) return rc;
   }
-  // Graph replay: with a detection every step, one step is a fixed sequence of kernels
-  // (propagation + keys, partition, pair scan, step advance).  It is captured once into a
-  // HIP graph whose kernels read their position from the device StepState, and replayed
-  // once per step.  No memset is captured: the partition kernel resets the scratch arena.
+  // Step graphs (ladds_config.step_graph, detection every step): see capture_step_graph.
+  // Otherwise the eager loop overlaps propagation + partition (stream) with the pair scan
+  // (stream2).  Both ping-pong between the two state buffers.
   const bool use_graph = ctx->cfg.step_graph != 0 && tpcd == 1 && ctx->variant == kVariantPlaneTiles &&
                          !ctx->profiling && ctx->n > 0 && nsteps > 0;
-  hipGraphExec_t graph_exec = nullptr;
-  if (use_graph) {
-    if (!ctx->d_step) HIPCHK(ctx, dalloc(&ctx->d_step, 1));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    hipGraph_t graph = nullptr;
-    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_propagate_steps(ctx, 0, 1, true, burnup_r2, ctx->d_step);
-    if (rc == LADDS_OK) rc = launch_detection(ctx, true, ctx->cfg.dt, digest, 0, ctx->d_step);
-    hipError_t e = rc == LADDS_OK ? launch_advance_step(ctx->d_step, ctx->stream) : hipSuccess;
-    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
-    if (rc != LADDS_OK) {
-      if (graph) hipGraphDestroy(graph);
-      return rc;
-    }
-    if (e == hipSuccess) e = ec;
-    if (e == hipSuccess) e = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
-    if (graph) hipGraphDestroy(graph);
-    if (e != hipSuccess) return ctx->hipfail(e, "step graph capture");
-  }
-  // Overlap of propagation + partition (stream) with the pair scan (stream2): the plane-tile
-  // scan launched eagerly.
   const bool use_overlap = !use_graph && ctx->variant == kVariantPlaneTiles && ctx->n > 0 && nsteps > 0;
   int cur = 0;                       // state buffer holding the latest state
   bool pending[2] = {false, false};  // a pair scan reading buffer b may still run on stream2
-  if (use_overlap) {
+  if (use_overlap || use_graph) {
     const int rc = ensure_overlap_buffers(ctx);
     if (rc != LADDS_OK) return rc;
+  }
+  hipGraphExec_t graphs[2] = {nullptr, nullptr};
+  auto drop_graphs = [&] {
+    for (hipGraphExec_t &g : graphs)
+      if (g) hipGraphExecDestroy(g);
+  };
+  if (use_graph) {
+    if (!ctx->d_step) HIPCHK(ctx, dalloc(&ctx->d_step, 1));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int b = 0; b < 2; ++b) {
+      const int rc = capture_step_graph(ctx, b, burnup_r2, digest, &graphs[b]);
+      if (rc != LADDS_OK) {
+        drop_graphs();
+        return rc;
+      }
+    }
   }
   for (int seg0 = 0; seg0 < nsteps && ctx->n > 0; seg0 += kEnvSegment) {
     const int seg = std::min(kEnvSegment, nsteps - seg0);
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // pinned environment buffer is free again
     if (use_graph) {
-      hipError_t e = launch_set_step(ctx->d_step, 0u, first_iteration + (uint64_t)seg0, ctx->stream);
-      for (int staged = 0; staged < seg && e == hipSuccess;) {
-        const int k = std::min(kChunk, seg - staged);
-        const int rc = stage_env(ctx, staged, k);
-        if (rc != LADDS_OK) {
-          hipGraphExecDestroy(graph_exec);
-          return rc;
+      // prologue: step 0 of the segment propagated and partitioned eagerly; steps j = 0..seg-2
+      // replay graph(buffer of step j): pair scan of step j beside step j+1's propagation;
+      // epilogue: the pair scan of the segment's last step.
+      const uint64_t base = first_iteration + (uint64_t)seg0;
+      int staged = 0;
+      auto stage_to = [&](int need) {  // environments of steps [0, need) staged
+        while (staged < need) {
+          const int k = std::min(kChunk, seg - staged);
+          const int rc = stage_env(ctx, staged, k);
+          if (rc != LADDS_OK) return rc;
+          staged += k;
         }
-        for (int q = 0; q < k && e == hipSuccess; ++q) e = hipGraphLaunch(graph_exec, ctx->stream);
-        staged += k;
+        return (int)LADDS_OK;
+      };
+      int rc = stage_to(std::min(seg, 2));
+      hipError_t e = hipSuccess;
+      if (rc == LADDS_OK) {
+        e = launch_set_step(ctx->d_step, 0u, base, ctx->stream);
+        if (e == hipSuccess) rc = launch_prop_part(ctx, cur, 0, 1, true, burnup_r2, nullptr, 0);
+        cur = 1 - cur;
       }
-      if (e != hipSuccess) {
-        hipGraphExecDestroy(graph_exec);
-        return ctx->hipfail(e, "step graph launch");
+      for (int j = 0; j + 1 < seg && rc == LADDS_OK && e == hipSuccess; ++j) {
+        rc = stage_to(std::min(seg, j + 2));
+        if (rc == LADDS_OK) e = hipGraphLaunch(graphs[cur], ctx->stream);
+        cur = 1 - cur;
+      }
+      if (rc == LADDS_OK && e == hipSuccess)
+        rc = launch_pairs_of(ctx, cur, ctx->cfg.dt, digest, base + (uint64_t)(seg - 1), nullptr, ctx->stream);
+      if (rc == LADDS_OK && e != hipSuccess) rc = ctx->hipfail(e, "step graph launch");
+      if (rc != LADDS_OK) {
+        drop_graphs();
+        return rc;
       }
       ctx->step_count += (uint64_t)seg;
       continue;
@@ -1289,9 +1341,10 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     }
     ctx->step_count += (uint64_t)seg;
   }
-  if (use_overlap) {
+  if (use_overlap || use_graph) {
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream2));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graphs();
     if (cur == 1) {  // the final state is in the second buffer: make it the context's own
       std::swap(ctx->d.rx, ctx->sb_r[0]);
       std::swap(ctx->d.ry, ctx->sb_r[1]);
@@ -1307,7 +1360,6 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   HIPCHK(ctx, hipMemcpyAsync(&sc, ctx->counters, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->log_count, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  if (graph_exec) hipGraphExecDestroy(graph_exec);
   if (sc.overflow) return ctx->fail(LADDS_E_OVERFLOW, "conjunction scan scratch arena exhausted");
   if (min_altitude_km > 0.0) {
     const int rc = compact_if_needed(ctx);

@@ -13,7 +13,7 @@ hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<d
                             const SubstepEnv<float> *env32, int nsteps, unsigned mask, const double cdt[4],
                             const double ddt[3], const GridParams *grid, uint32_t *keys, uint32_t *count,
                             uint32_t *slot, double burnup_r2, const StepState *step_state, hipStream_t stream,
-                            const DeviceObjects *out = nullptr);
+                            const DeviceObjects *out = nullptr, uint32_t step_offset = 0);
 // Graph replay bookkeeping: set the step position, advance it by one step.
 hipError_t launch_set_step(StepState *st, uint32_t step, uint64_t iter_base, hipStream_t stream);
 hipError_t launch_advance_step(StepState *st, hipStream_t stream);

@@ -35,7 +35,8 @@ struct PropagateArgs {
   uint32_t *count;  // non-null: counting pass of the atomic counting sort
   uint32_t *slot;
   double burnup_r2;  // > 0: objects with |r|^2 below become dummies (deleteBurnUps)
-  const StepState *step_state;  // non-null (graph replay): environments start at step_state->step
+  const StepState *step_state;  // non-null (graph replay): environments start at step_state->step + step_offset
+  uint32_t step_offset;
   // output state (the input arrays of `o` when propagating in place)
   double *orx, *ory, *orz, *ovx, *ovy, *ovz;
   int32_t *oact;
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
   }
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.o.n) return;
-  const size_t env0 = a.step_state ? 3u * (size_t)a.step_state->step : 0u;
+  const size_t env0 = a.step_state ? 3u * ((size_t)a.step_state->step + a.step_offset) : 0u;
   double x = a.o.rx[i], y = a.o.ry[i], z = a.o.rz[i];
   double vx = a.o.vx[i], vy = a.o.vy[i], vz = a.o.vz[i];
   ObjParams p;
@@ -121,7 +122,7 @@ hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<d
                             const SubstepEnv<float> *env32, int nsteps, unsigned mask, const double cdt[4],
                             const double ddt[3], const GridParams *grid, uint32_t *keys, uint32_t *count,
                             uint32_t *slot, double burnup_r2, const StepState *step_state, hipStream_t stream,
-                            const DeviceObjects *out) {
+                            const DeviceObjects *out, uint32_t step_offset) {
   if (o.n == 0 || nsteps <= 0) return hipSuccess;
   PropagateArgs a;
   a.o = o;
@@ -145,6 +146,7 @@ hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<d
   a.slot = slot;
   a.burnup_r2 = burnup_r2;
   a.step_state = step_state;
+  a.step_offset = step_offset;
   const unsigned blocks = (unsigned)((o.n + 255) / 256);
   switch (form) {
     case kFormFp64:
