@@ -55,6 +55,19 @@ BYTES = {
 }
 
 
+# HBM bytes per launch at 1,048,576 objects (read + write), measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE scaled by a 256 MiB copy kernel, default plane-tile shape; the
+# runs and the commands are in profiles/hbm_traffic_1M_full_model.md.
+MEASURED_TRAFFIC_1M = {
+    ("propagate", "fp64"): (71.46 + 58.72) * 1e6,
+    ("propagate", "split"): (67.24 + 58.72) * 1e6,
+    ("tile_partition", "fp64"): (29.44 + 48.08) * 1e6,
+    ("tile_partition", "split"): (29.44 + 17.83) * 1e6,
+    ("pair_tiles", "fp64"): (100.26 + 0.06) * 1e6,
+    ("pair_tiles", "split"): (69.86 + 0.06) * 1e6,
+}
+
+
 def parse():
     p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     p.add_argument("--gpus", type=int, default=1)
@@ -365,15 +378,24 @@ def main():
         ks, kf = pairkeys(res["split"]["log"]), pairkeys(f["log"])
         out["pairs_equal"] = bool(ks.shape == kf.shape and np.array_equal(ks, kf))
         out["pairs"] = {"split": int(ks.shape[0]), "fp64": int(kf.shape[0])}
-    # roofline of the dominant kernel of the headline form
+    # roofline of the dominant kernel of the headline form (and of every kernel, per form)
+    def roof(form, name, k):
+        per_obj = BYTES.get(name, {}).get(form)
+        if not per_obj:
+            return None
+        achieved = per_obj * a.objects / (k["avg_us"] * 1e-6) / 1e9
+        traffic = MEASURED_TRAFFIC_1M.get((name, form)) if a.objects == 1 << 20 else None
+        return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_us": k["avg_us"],
+                "bytes_per_object": per_obj}
+
     kern = m["kernels"]
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
-    per_obj = BYTES.get(dom, {}).get(main_form)
-    if per_obj:
-        achieved = per_obj * a.objects / (kern[dom]["avg_us"] * 1e-6) / 1e9
-        out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                           "avg_us": kern[dom]["avg_us"], "bytes_per_object": per_obj}
+    r = roof(main_form, dom, kern[dom])
+    if r:
+        out["roofline"] = r
+    out["roofline_all"] = {form: {name: roof(form, name, k) for name, k in res[form]["kernels"].items()}
+                           for form in res}
     out["tuning"] = {form: res[form]["tuning"] for form in res}
     out["kernels_us"] = {form: {k: round(v["avg_us"], 2) for k, v in res[form]["kernels"].items()} for form in res}
     grads = None
