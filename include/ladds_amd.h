@@ -171,6 +171,15 @@ LADDS_API int ladds_configure(ladds_ctx *ctx, const ladds_config *cfg);
 /* AutoPas::addParticle for a whole population (SatelliteLoader.cpp:17-84). Replaces all objects. */
 LADDS_API int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *objs);
 LADDS_API size_t ladds_num_objects(const ladds_ctx *ctx);
+/* Breakup hook, part 1: append objects to the current population, keeping its state --
+ * BreakupWrapper.cpp:38-43 (particle/) adds each fragment with
+ * autopas.addParticle(..., parentID).  Fragments of one breakup share a parent id, and the
+ * predicate skips pairs with equal parent ids (CollisionFunctor.cpp:65-69).  Ids default
+ * to the object's index.  Not for ensembles (LADDS_E_STATE). */
+LADDS_API int ladds_add_objects(ladds_ctx *ctx, const ladds_objects *objs);
+/* Breakup hook, part 2: remove the objects with the given ids (order of the rest kept) --
+ * autopas.deleteParticle of the two colliding objects, BreakupWrapper.cpp:23-25. */
+LADDS_API int ladds_delete_objects(ladds_ctx *ctx, const uint64_t *ids, size_t count, size_t *removed);
 
 /* Batched ensembles (no LADDS counterpart; LADDS runs one scenario per process): the
  * objects last set are the concatenation of independent scenarios, scenario s occupying

@@ -12,6 +12,8 @@
  *   YoshidaIntegrator (:145, integrate() at :297)          ladds_amd::YoshidaIntegrator
  *   CollisionFunctor (CollisionFunctor.h:17-110)           ladds_amd::CollisionFunctor
  *   AutoPas::iteratePairwise (Simulation.cpp:220)          Container::iteratePairwise
+ *   DatasetReader::readDataset (io/DatasetReader.cpp)      ladds_amd::DatasetReader
+ *   ConjunctionLogger (io/ConjuctionLogger.cpp)            ladds_amd::ConjunctionLogger
  *
  * Ownership follows the reference: the container owns the objects (on the device, with a
  * host mirror that is refreshed on demand); the accumulator, integrator and functor hold
@@ -23,9 +25,12 @@
 #define LADDS_AMD_HPP
 
 #include <array>
+#include <charconv>
 #include <cmath>
 #include <cstdint>
+#include <fstream>
 #include <limits>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -53,11 +58,12 @@ class Particle {
   enum class ActivityState : int { passive = LADDS_PASSIVE, evasive = LADDS_EVASIVE, evasivePreserving = LADDS_EVASIVE_PRESERVING };
 
   /* Particle.cpp:13-27: bc_inv from c_D and the cross-section when B* is absent or 0,
-   * otherwise from B* (given in Earth radii^-1) as 2 R_E / 0.1570 * B* (R_E in m). */
+   * otherwise 2 R_E / 0.1570 * B*.  R_E is the propagator's Earth radius in km, the unit
+   * Simulation.cpp:573 adds it to a km altitude in. */
   static double calculateBcInv(double bstar, double mass, double radius, double coefficientOfDrag) {
     if (std::isnan(bstar) || bstar == 0.0) return coefficientOfDrag * (M_PI * radius * radius) / mass;
-    const double earth_radius_m = 6378.1363e3;
-    return (2.0 * earth_radius_m / 0.1570) * bstar;
+    const double earth_radius_km = 6378.1363;
+    return (2.0 * earth_radius_km / 0.1570) * bstar;
   }
 
   /* Same argument order as LADDS::Particle's constructor (Particle.h:60-80); cr is the
@@ -131,7 +137,15 @@ class Container {
   void addParticle(const Particle &p) {
     syncHost();
     particles_.push_back(p);
-    device_dirty_ = true;
+    ++appended_;  // appended on the device (ladds_add_objects) unless a full upload is due
+  }
+  /* autopas.deleteParticle (BreakupWrapper.cpp:24-25): removed on the device, the mirror
+   * follows at its next download; Particle* into the mirror stay valid until then. */
+  void deleteParticle(const Particle &p) {
+    syncDevice();
+    const uint64_t id = p.getID();
+    check(ladds_delete_objects(ctx_, &id, 1, nullptr), ctx_, "ladds_delete_objects");
+    host_stale_ = true;
   }
   size_t getNumberOfParticles() { syncHost(); return particles_.size(); }
   /* Host view of the objects; reading it after device work downloads the state first. */
@@ -178,26 +192,14 @@ class Container {
   ladds_ctx *context() { return ctx_; }
   ladds_config &config() { return cfg_; }
   void applyConfig() { check(ladds_configure(ctx_, &cfg_), ctx_, "ladds_configure"); }
-  /* Upload the host mirror if it was changed since the last upload. */
+  /* Upload the host mirror if it was changed since the last upload: the whole mirror after
+   * host-side edits, only the new tail after addParticle calls (breakup fragments). */
   void syncDevice() {
-    if (!device_dirty_) return;
     const size_t n = particles_.size();
-    std::vector<double> col(11 * n);
-    std::vector<int32_t> act(n);
-    std::vector<uint64_t> id(n), parent(n);
-    double *rx = col.data(), *ry = rx + n, *rz = ry + n, *vx = rz + n, *vy = vx + n, *vz = vy + n;
-    double *rad = vz + n, *aom = rad + n, *bc = aom + n, *cr = bc + n;
-    for (size_t i = 0; i < n; ++i) {
-      const Particle &p = particles_[i];
-      rx[i] = p.getPosition()[0]; ry[i] = p.getPosition()[1]; rz[i] = p.getPosition()[2];
-      vx[i] = p.getVelocity()[0]; vy[i] = p.getVelocity()[1]; vz[i] = p.getVelocity()[2];
-      rad[i] = p.getRadius(); aom[i] = p.getAom(); bc[i] = p.getBcInv(); cr[i] = p.getCr();
-      act[i] = static_cast<int32_t>(p.getActivityState());
-      id[i] = p.getID(); parent[i] = p.getParentIdentifier();
-    }
-    ladds_objects o{n, rx, ry, rz, vx, vy, vz, rad, aom, bc, cr, act.data(), id.data(), parent.data()};
-    check(ladds_set_objects(ctx_, &o), ctx_, "ladds_set_objects");
+    if (device_dirty_) upload(0, n, ladds_set_objects, "ladds_set_objects");
+    else if (appended_) upload(n - appended_, n, ladds_add_objects, "ladds_add_objects");
     device_dirty_ = false;
+    appended_ = 0;
     host_stale_ = false;
   }
   /* Download the device state into the host mirror if the device moved ahead of it.
@@ -236,6 +238,26 @@ class Container {
   std::vector<Particle> particles_;
   bool device_dirty_ = false;
   bool host_stale_ = false;
+  size_t appended_ = 0;
+
+  void upload(size_t b, size_t e, int (*entry)(ladds_ctx *, const ladds_objects *), const char *what) {
+    const size_t n = e - b;
+    std::vector<double> col(10 * n);
+    std::vector<int32_t> act(n);
+    std::vector<uint64_t> id(n), parent(n);
+    double *rx = col.data(), *ry = rx + n, *rz = ry + n, *vx = rz + n, *vy = vx + n, *vz = vy + n;
+    double *rad = vz + n, *aom = rad + n, *bc = aom + n, *cr = bc + n;
+    for (size_t i = 0; i < n; ++i) {
+      const Particle &p = particles_[b + i];
+      rx[i] = p.getPosition()[0]; ry[i] = p.getPosition()[1]; rz[i] = p.getPosition()[2];
+      vx[i] = p.getVelocity()[0]; vy[i] = p.getVelocity()[1]; vz[i] = p.getVelocity()[2];
+      rad[i] = p.getRadius(); aom[i] = p.getAom(); bc[i] = p.getBcInv(); cr[i] = p.getCr();
+      act[i] = static_cast<int32_t>(p.getActivityState());
+      id[i] = p.getID(); parent[i] = p.getParentIdentifier();
+    }
+    ladds_objects o{n, rx, ry, rz, vx, vy, vz, rad, aom, bc, cr, act.data(), id.data(), parent.data()};
+    check(entry(ctx_, &o), ctx_, what);
+  }
 };
 
 namespace Acceleration {
@@ -342,6 +364,77 @@ inline bool Container::iteratePairwise(CollisionFunctor *f) {
   convert(evaded, f->evaded_);
   return false;
 }
+
+/* DatasetReader::readDataset (src/ladds/io/DatasetReader.cpp:15-50): a CSV file with one
+ * header line and the columns id, COSPAR id, name, B* [1/Earth radius], mass [kg],
+ * radius [m], activity (passive | evasive | evasivePreserving, Particle.cpp:130-141),
+ * r x y z [km], v x y z [km/s].  bc_inv follows Particle::calculateBcInv with the given
+ * drag coefficient; parent ids are "none" (max size_t). */
+class DatasetReader {
+ public:
+  static std::vector<Particle> readDataset(const std::string &csvFilepath, double coefficientOfDrag) {
+    std::ifstream in(csvFilepath);
+    if (!in) throw std::runtime_error("cannot open dataset " + csvFilepath);
+    std::string line;
+    std::getline(in, line);  // header
+    std::vector<Particle> out;
+    size_t lineno = 1;
+    while (std::getline(in, line)) {
+      ++lineno;
+      if (line.empty() || line == "\r") continue;
+      if (!line.empty() && line.back() == '\r') line.pop_back();
+      std::vector<std::string> f;
+      std::stringstream ls(line);
+      std::string cell;
+      while (std::getline(ls, cell, ',')) f.push_back(cell);
+      if (f.size() != 13) throw std::runtime_error(csvFilepath + ":" + std::to_string(lineno) + ": expected 13 columns");
+      auto num = [&](size_t k) {
+        try {
+          return std::stod(f[k]);
+        } catch (const std::exception &) {
+          throw std::runtime_error(csvFilepath + ":" + std::to_string(lineno) + ": bad number '" + f[k] + "'");
+        }
+      };
+      Particle::ActivityState act = Particle::ActivityState::passive;
+      if (f[6] == "evasive") act = Particle::ActivityState::evasive;
+      if (f[6] == "evasivePreserving") act = Particle::ActivityState::evasivePreserving;
+      const double bstar = num(3), mass = num(4), radius = num(5);
+      out.emplace_back(std::array<double, 3>{num(7), num(8), num(9)}, std::array<double, 3>{num(10), num(11), num(12)},
+                       static_cast<size_t>(std::stoull(f[0])), f[1], act, mass, radius,
+                       Particle::calculateBcInv(bstar, mass, radius, coefficientOfDrag),
+                       std::numeric_limits<size_t>::max());
+    }
+    return out;
+  }
+};
+
+/* ConjunctionLogger (src/ladds/io/ConjuctionLogger.cpp:13-44): CSV with the header
+ * "Iteration,P1,P2,SquaredDistance" and one line per pair (smaller id first), doubles in
+ * their shortest round-trip form.  Collisions and evasions go to the same file, as in the
+ * reference. */
+class ConjunctionLogger {
+ public:
+  explicit ConjunctionLogger(const std::string &path) : out_(path) {
+    if (!out_) throw std::runtime_error("cannot open " + path);
+    out_ << "Iteration,P1,P2,SquaredDistance\n";
+  }
+  void log(uint64_t iteration, uint64_t id1, uint64_t id2, double distanceSquared) {
+    char buf[32];
+    const auto r = std::to_chars(buf, buf + sizeof(buf), distanceSquared);
+    out_ << iteration << ',' << id1 << ',' << id2 << ',' << std::string(buf, r.ptr) << '\n';
+  }
+  void writeConjunctions(uint64_t iteration, const std::vector<std::tuple<Particle *, Particle *, double,
+                                                                           std::array<double, 3>>> &collisions) {
+    for (const auto &c : collisions) log(iteration, std::get<0>(c)->getID(), std::get<1>(c)->getID(), std::get<2>(c));
+  }
+  /* Pairs logged on the device by Container::simulate. */
+  void writeLog(const std::vector<ladds_logged_pair> &entries) {
+    for (const ladds_logged_pair &e : entries) log(e.iteration, e.pair.id_a, e.pair.id_b, e.pair.d2);
+  }
+
+ private:
+  std::ofstream out_;
+};
 
 }  // namespace ladds_amd
 

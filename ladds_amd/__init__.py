@@ -29,7 +29,8 @@ from ._abi import (ALL_COMPONENTS, C22, DRAG, EVASIVE, EVASIVE_PRESERVING, FP64,
                    THRESHOLD_RADIUS_SCALED, LOGGED_DTYPE, PAIR_DTYPE)
 
 __all__ = [
-    "Engine", "EngineConfig", "LaddsError", "generate_population", "generate_ensemble", "device_count",
+    "Engine", "EngineConfig", "LaddsError", "generate_population", "generate_ensemble", "read_dataset",
+    "write_conjunctions_csv", "bc_inv_from", "device_count",
     "components_mask", "KEP", "J2", "C22", "S22", "SOL", "LUN", "SRP", "DRAG", "ALL_COMPONENTS", "FP64", "SPLIT",
     "SPLIT_ALL32",
     "THRESHOLD_RADIUS_SCALED", "THRESHOLD_ABSOLUTE", "PASSIVE", "EVASIVE", "EVASIVE_PRESERVING",
@@ -142,9 +143,61 @@ def generate_population(workload: int, seed: int, n: int, planted_pairs: int = 0
     return pop
 
 
+_ACTIVITY = {"passive": PASSIVE, "evasive": EVASIVE, "evasivePreserving": EVASIVE_PRESERVING}
+_R_EARTH_KM = 6378.1363
+
+
+def bc_inv_from(bstar, mass, radius, coefficient_of_drag):
+    """``Particle::calculateBcInv`` (src/ladds/particle/Particle.cpp:13-27): c_D A / m when B* is
+    NaN or 0, else 2 R_E / 0.1570 B* with R_E in km (the unit src/ladds/Simulation.cpp:573 adds it
+    to a km altitude in)."""
+    bstar, mass, radius = (np.asarray(x, np.float64) for x in (bstar, mass, radius))
+    by_cd = coefficient_of_drag * (np.pi * radius * radius) / mass
+    return np.where(np.isnan(bstar) | (bstar == 0.0), by_cd, (2.0 * _R_EARTH_KM / 0.1570) * bstar)
+
+
+def read_dataset(path, coefficient_of_drag: float = 2.2):
+    """``DatasetReader::readDataset`` (src/ladds/io/DatasetReader.cpp:15-50): CSV with a header
+    and the columns id, COSPAR id, name, B*, mass [kg], radius [m], activity, r [km], v [km/s].
+    Returns the Population and the COSPAR identifiers."""
+    import csv
+
+    rows = []
+    with open(path, newline="") as f:
+        rd = csv.reader(f)
+        next(rd)
+        for lineno, row in enumerate(rd, start=2):
+            if not row:
+                continue
+            if len(row) != 13:
+                raise ValueError(f"{path}:{lineno}: expected 13 columns, got {len(row)}")
+            rows.append(row)
+    n = len(rows)
+    col = lambda k: np.array([float(r[k]) for r in rows], np.float64)
+    mass, radius, bstar = col(4), col(5), col(3)
+    pop = Population(rx=col(7), ry=col(8), rz=col(9), vx=col(10), vy=col(11), vz=col(12), radius_m=radius,
+                     mass_kg=mass, aom=np.pi * radius * radius * 1e-6 / mass,
+                     bc_inv=bc_inv_from(bstar, mass, radius, coefficient_of_drag), cr=np.ones(n))
+    pop.activity = np.array([_ACTIVITY.get(r[6], PASSIVE) for r in rows], np.int32)
+    pop.id = np.array([int(r[0]) for r in rows], np.uint64)
+    pop.parent = np.full(n, np.iinfo(np.uint64).max, np.uint64)
+    return pop, [r[1] for r in rows]
+
+
+def write_conjunctions_csv(path, log):
+    """``ConjunctionLogger`` format (src/ladds/io/ConjuctionLogger.cpp:13-44): header
+    ``Iteration,P1,P2,SquaredDistance``, one line per logged pair (smaller id first), the
+    squared distance in its shortest round-trip form."""
+    with open(path, "w") as f:
+        f.write("Iteration,P1,P2,SquaredDistance\n")
+        for e in log:
+            f.write(f"{int(e['iteration'])},{int(e['pair']['id_a'])},{int(e['pair']['id_b'])},"
+                    f"{float(e['pair']['d2'])!r}\n")
+
+
 def generate_ensemble(sizes, seed: int, workload: int = 1, planted_pairs: int = 0):
     """Concatenated seeded scenarios (scenario s: ``generate_population(workload, seed + s, sizes[s],
-    planted_pairs)``, i.e. sizes[s] + 2 planted_pairs objects) and their offsets.  Ids are
+    planted_pairs)``: sizes[s] objects, planted partners included) and their offsets.  Ids are
     ``(s << 32) | local index``, so pairs name their scenario."""
     parts = [generate_population(workload, seed + s, int(k), planted_pairs) for s, k in enumerate(sizes)]
     sizes = [p.n for p in parts]
@@ -200,8 +253,8 @@ class Engine:
         self._check(lib.ladds_configure(self._h, C.byref(self._c)), "configure")
 
     # -- state ----------------------------------------------------------
-    def set_objects(self, pop: Population | None = None, **arrays):
-        """``AutoPas::addParticle`` for a whole population (replaces all objects)."""
+    @staticmethod
+    def _pack(pop, arrays):
         src = dict(pop.__dict__) if pop is not None else {}
         src.update(arrays)
         n = len(src["rx"])
@@ -222,7 +275,20 @@ class Engine:
             if src.get(name) is not None:
                 keep[name] = np.ascontiguousarray(src[name], dtype=np.uint64)
                 setattr(o, name, _ptr(keep[name], C.c_uint64))
+        return o, keep
+
+    def set_objects(self, pop: Population | None = None, **arrays):
+        """``AutoPas::addParticle`` for a whole population (replaces all objects)."""
+        o, keep = self._pack(pop, arrays)
         self._check(lib.ladds_set_objects(self._h, C.byref(o)), "set_objects")
+
+    def add_objects(self, pop: Population | None = None, **arrays):
+        """Append objects and keep the current state: the breakup hook
+        (particle/BreakupWrapper.cpp:38-43 adds each fragment to the container).
+        Fragments of one breakup share ``parent``; pairs with equal parent ids
+        are skipped (CollisionFunctor.cpp:65-69)."""
+        o, keep = self._pack(pop, arrays)
+        self._check(lib.ladds_add_objects(self._h, C.byref(o)), "add_objects")
 
     @property
     def n(self) -> int:
@@ -300,6 +366,15 @@ class Engine:
     def delete_burnups(self, min_altitude_km: float) -> int:
         removed = C.c_size_t(0)
         self._check(lib.ladds_delete_burnups(self._h, float(min_altitude_km), C.byref(removed)), "delete_burnups")
+        return removed.value
+
+    def delete_objects(self, ids) -> int:
+        """Remove the objects with these ids, order of the rest kept (``autopas.deleteParticle``
+        of the colliding pair, BreakupWrapper.cpp:23-25)."""
+        a = np.ascontiguousarray(np.atleast_1d(ids), dtype=np.uint64)
+        removed = C.c_size_t(0)
+        self._check(lib.ladds_delete_objects(self._h, _ptr(a, C.c_uint64), a.size, C.byref(removed)),
+                    "delete_objects")
         return removed.value
 
     def set_scenarios(self, offsets):

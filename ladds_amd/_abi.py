@@ -149,6 +149,8 @@ _SIGNATURES = {
     "ladds_configure": (C.c_int, [_ctx, C.POINTER(Config)]),
     "ladds_set_objects": (C.c_int, [_ctx, C.POINTER(Objects)]),
     "ladds_num_objects": (C.c_size_t, [_ctx]),
+    "ladds_add_objects": (C.c_int, [_ctx, C.POINTER(Objects)]),
+    "ladds_delete_objects": (C.c_int, [_ctx, C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(C.c_size_t)]),
     "ladds_set_scenarios": (C.c_int, [_ctx, C.c_size_t, C.POINTER(C.c_uint64)]),
     "ladds_num_scenarios": (C.c_size_t, [_ctx]),
     "ladds_get_scenarios": (C.c_int, [_ctx, C.POINTER(C.c_uint64)]),

@@ -14,6 +14,7 @@
 #include <map>
 #include <string>
 #include <array>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/ladds_amd.h"
@@ -859,34 +860,23 @@ int ladds_configure(ladds_ctx *ctx, const ladds_config *cfg) {
   return ensure_log(ctx, std::max<uint64_t>(log_cap, 1024));
 }
 
-int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
-  if (!ctx || !o) return LADDS_E_INVALID;
+}  // extern "C"
+
+namespace {
+
+int check_objects(ladds_ctx *ctx, const ladds_objects *o) {
   if (o->n > 0 && (!o->rx || !o->ry || !o->rz || !o->vx || !o->vy || !o->vz || !o->radius_m))
     return ctx->fail(LADDS_E_INVALID, "positions, velocities and radius_m are required");
-  if (o->n >= (size_t)UINT32_MAX) return ctx->fail(LADDS_E_INVALID, "at most 2^32-2 objects per context");
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (o->activity)
+    for (size_t i = 0; i < o->n; ++i)
+      if (o->activity[i] < 0 || o->activity[i] > 2) return ctx->fail(LADDS_E_INVALID, "activity must be 0, 1 or 2");
+  return LADDS_OK;
+}
+
+// Host preparation (derived coefficients, defaults) and upload of objects into
+// [offset, offset + o->n) of the device arrays.
+int upload_objects(ladds_ctx *ctx, const ladds_objects *o, size_t offset) {
   const size_t n = o->n;
-  if (n > ctx->cap) {
-    free_objects(ctx->d);
-    free_objects(ctx->d_alt);
-    hipFree(ctx->slot);
-      hipFree(ctx->rec);
-    HIPCHK(ctx, alloc_objects(ctx->d, n));
-    HIPCHK(ctx, dalloc(&ctx->slot, n));
-    hipFree(ctx->scratch);
-    ctx->scratch = nullptr;  // the scan's scratch arena is sized by setup_grid
-    ctx->scratch_bytes = 0;
-    HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&ctx->rec), n));
-    ctx->cap = n;
-  }
-  {
-    ctx->scen_off.clear();  // a new population is a single scenario until ladds_set_scenarios
-    const int rc = setup_grid(ctx, n, o->rx);
-    if (rc != LADDS_OK) return rc;
-  }
-  ctx->n = n;
-  ctx->d.n = n;
   if (n == 0) return LADDS_OK;
   std::vector<double> radius(o->radius_m, o->radius_m + n), aom(n, 0.0), bc(n, 0.0), cr(n, 1.0), srp64(n),
       drag64(n);
@@ -898,36 +888,129 @@ int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
     if (o->bc_inv) bc[i] = o->bc_inv[i];
     if (o->cr) cr[i] = o->cr[i];
     if (o->activity) act[i] = o->activity[i];
-    id[i] = o->id ? o->id[i] : (uint64_t)i;
+    id[i] = o->id ? o->id[i] : (uint64_t)(offset + i);
     if (o->parent) parent[i] = o->parent[i];
-    if (act[i] < 0 || act[i] > 2) return ctx->fail(LADDS_E_INVALID, "activity must be 0, 1 or 2");
     srp64[i] = kSrpCoef * cr[i] * aom[i];
     drag64[i] = kDragCoef * bc[i];
     srp32[i] = (float)srp64[i];
     drag32[i] = (float)drag64[i];
   }
-  auto up = [&](void *dst, const void *src, size_t bytes) {
-    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+  auto up = [&](auto *dst, const void *src, size_t elem) {
+    return hipMemcpyAsync(dst + offset, src, elem * n, hipMemcpyHostToDevice, ctx->stream);
   };
-  HIPCHK(ctx, up(ctx->d.rx, o->rx, 8 * n));
-  HIPCHK(ctx, up(ctx->d.ry, o->ry, 8 * n));
-  HIPCHK(ctx, up(ctx->d.rz, o->rz, 8 * n));
-  HIPCHK(ctx, up(ctx->d.vx, o->vx, 8 * n));
-  HIPCHK(ctx, up(ctx->d.vy, o->vy, 8 * n));
-  HIPCHK(ctx, up(ctx->d.vz, o->vz, 8 * n));
-  HIPCHK(ctx, up(ctx->d.radius_m, radius.data(), 8 * n));
-  HIPCHK(ctx, up(ctx->d.aom, aom.data(), 8 * n));
-  HIPCHK(ctx, up(ctx->d.bc_inv, bc.data(), 8 * n));
-  HIPCHK(ctx, up(ctx->d.cr, cr.data(), 8 * n));
-  HIPCHK(ctx, up(ctx->d.srp64, srp64.data(), 8 * n));
-  HIPCHK(ctx, up(ctx->d.drag64, drag64.data(), 8 * n));
-  HIPCHK(ctx, up(ctx->d.srp32, srp32.data(), 4 * n));
-  HIPCHK(ctx, up(ctx->d.drag32, drag32.data(), 4 * n));
-  HIPCHK(ctx, up(ctx->d.activity, act.data(), 4 * n));
-  HIPCHK(ctx, up(ctx->d.id, id.data(), 8 * n));
-  HIPCHK(ctx, up(ctx->d.parent, parent.data(), 8 * n));
+  HIPCHK(ctx, up(ctx->d.rx, o->rx, 8));
+  HIPCHK(ctx, up(ctx->d.ry, o->ry, 8));
+  HIPCHK(ctx, up(ctx->d.rz, o->rz, 8));
+  HIPCHK(ctx, up(ctx->d.vx, o->vx, 8));
+  HIPCHK(ctx, up(ctx->d.vy, o->vy, 8));
+  HIPCHK(ctx, up(ctx->d.vz, o->vz, 8));
+  HIPCHK(ctx, up(ctx->d.radius_m, radius.data(), 8));
+  HIPCHK(ctx, up(ctx->d.aom, aom.data(), 8));
+  HIPCHK(ctx, up(ctx->d.bc_inv, bc.data(), 8));
+  HIPCHK(ctx, up(ctx->d.cr, cr.data(), 8));
+  HIPCHK(ctx, up(ctx->d.srp64, srp64.data(), 8));
+  HIPCHK(ctx, up(ctx->d.drag64, drag64.data(), 8));
+  HIPCHK(ctx, up(ctx->d.srp32, srp32.data(), 4));
+  HIPCHK(ctx, up(ctx->d.drag32, drag32.data(), 4));
+  HIPCHK(ctx, up(ctx->d.activity, act.data(), 4));
+  HIPCHK(ctx, up(ctx->d.id, id.data(), 8));
+  HIPCHK(ctx, up(ctx->d.parent, parent.data(), 8));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return LADDS_OK;
+}
+
+// Object capacity of at least `need`; with `keep` the first ctx->n objects are preserved.
+int reserve_objects(ladds_ctx *ctx, size_t need, bool keep) {
+  if (need <= ctx->cap) return LADDS_OK;
+  const size_t cap = keep ? std::max(need, 2 * ctx->cap) : need;
+  DeviceObjects fresh{};
+  const hipError_t ae = alloc_objects(fresh, cap);
+  if (ae != hipSuccess) {
+    free_objects(fresh);
+    return ctx->hipfail(ae, "alloc_objects");
+  }
+  if (keep && ctx->n) {
+    const size_t n = ctx->n;
+    auto cp = [&](void *dst, const void *src, size_t elem) {
+      return hipMemcpyAsync(dst, src, elem * n, hipMemcpyDeviceToDevice, ctx->stream);
+    };
+    const DeviceObjects &o = ctx->d;
+    HIPCHK(ctx, cp(fresh.rx, o.rx, 8));
+    HIPCHK(ctx, cp(fresh.ry, o.ry, 8));
+    HIPCHK(ctx, cp(fresh.rz, o.rz, 8));
+    HIPCHK(ctx, cp(fresh.vx, o.vx, 8));
+    HIPCHK(ctx, cp(fresh.vy, o.vy, 8));
+    HIPCHK(ctx, cp(fresh.vz, o.vz, 8));
+    HIPCHK(ctx, cp(fresh.radius_m, o.radius_m, 8));
+    HIPCHK(ctx, cp(fresh.aom, o.aom, 8));
+    HIPCHK(ctx, cp(fresh.bc_inv, o.bc_inv, 8));
+    HIPCHK(ctx, cp(fresh.cr, o.cr, 8));
+    HIPCHK(ctx, cp(fresh.srp64, o.srp64, 8));
+    HIPCHK(ctx, cp(fresh.drag64, o.drag64, 8));
+    HIPCHK(ctx, cp(fresh.srp32, o.srp32, 4));
+    HIPCHK(ctx, cp(fresh.drag32, o.drag32, 4));
+    HIPCHK(ctx, cp(fresh.activity, o.activity, 4));
+    HIPCHK(ctx, cp(fresh.id, o.id, 8));
+    HIPCHK(ctx, cp(fresh.parent, o.parent, 8));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  fresh.n = ctx->d.n;
+  free_objects(ctx->d);
+  free_objects(ctx->d_alt);
+  ctx->d = fresh;
+  hipFree(ctx->slot);
+  hipFree(ctx->rec);
+  hipFree(ctx->scratch);
+  ctx->slot = nullptr;
+  ctx->rec = nullptr;
+  ctx->scratch = nullptr;  // the scan's scratch arena is sized by setup_grid
+  ctx->scratch_bytes = 0;
+  HIPCHK(ctx, dalloc(&ctx->slot, cap));
+  HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&ctx->rec), cap));
+  ctx->cap = cap;
+  return LADDS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
+  if (!ctx || !o) return LADDS_E_INVALID;
+  if (o->n >= (size_t)UINT32_MAX) return ctx->fail(LADDS_E_INVALID, "at most 2^32-2 objects per context");
+  int rc = check_objects(ctx, o);
+  if (rc != LADDS_OK) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const size_t n = o->n;
+  rc = reserve_objects(ctx, n, false);
+  if (rc != LADDS_OK) return rc;
+  ctx->scen_off.clear();  // a new population is a single scenario until ladds_set_scenarios
+  rc = setup_grid(ctx, n, o->rx);
+  if (rc != LADDS_OK) return rc;
+  ctx->n = n;
+  ctx->d.n = n;
+  return upload_objects(ctx, o, 0);
+}
+
+int ladds_add_objects(ladds_ctx *ctx, const ladds_objects *o) {
+  if (!ctx || !o) return LADDS_E_INVALID;
+  if (ctx->scen_off.size() > 2) return ctx->fail(LADDS_E_STATE, "objects cannot be added to an ensemble");
+  if (ctx->n + o->n >= (size_t)UINT32_MAX) return ctx->fail(LADDS_E_INVALID, "at most 2^32-2 objects per context");
+  int rc = check_objects(ctx, o);
+  if (rc != LADDS_OK || o->n == 0) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const size_t n0 = ctx->n, n = n0 + o->n;
+  rc = reserve_objects(ctx, n, true);
+  if (rc != LADDS_OK) return rc;
+  rc = upload_objects(ctx, o, n0);
+  if (rc != LADDS_OK) return rc;
+  ctx->n = n;
+  ctx->d.n = n;
+  std::vector<double> rx(n);
+  HIPCHK(ctx, hipMemcpy(rx.data(), ctx->d.rx, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return setup_grid(ctx, n, rx.data());
 }
 
 int ladds_set_scenarios(ladds_ctx *ctx, size_t nscenarios, const uint64_t *offsets) {
@@ -1428,6 +1511,33 @@ int ladds_delete_burnups(ladds_ctx *ctx, double min_altitude_km, size_t *removed
     HIPCHK(ctx, hipMemcpyAsync(ctx->d.activity, act.data(), 4 * before, hipMemcpyHostToDevice, ctx->stream));
     const int rc = compact_if_needed(ctx);
     if (rc != LADDS_OK) return rc;
+  }
+  if (removed) *removed = before - ctx->n;
+  return LADDS_OK;
+}
+
+int ladds_delete_objects(ladds_ctx *ctx, const uint64_t *ids, size_t count, size_t *removed) {
+  if (!ctx || (count && !ids)) return LADDS_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t before = ctx->n;
+  if (before && count) {
+    std::unordered_set<uint64_t> doomed(ids, ids + count);
+    std::vector<uint64_t> id(before);
+    std::vector<int32_t> act(before);
+    HIPCHK(ctx, hipMemcpyAsync(id.data(), ctx->d.id, 8 * before, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(act.data(), ctx->d.activity, 4 * before, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    bool any = false;
+    for (size_t i = 0; i < before; ++i)
+      if (doomed.count(id[i])) {
+        act[i] = -1;
+        any = true;
+      }
+    if (any) {
+      HIPCHK(ctx, hipMemcpyAsync(ctx->d.activity, act.data(), 4 * before, hipMemcpyHostToDevice, ctx->stream));
+      const int rc = compact_if_needed(ctx);
+      if (rc != LADDS_OK) return rc;
+    }
   }
   if (removed) *removed = before - ctx->n;
   return LADDS_OK;

@@ -14,6 +14,9 @@
 //   state                                                         -> "state id rx ry rz vx vy vz"
 //   burnups <min_altitude_km>                                     -> "removed n"
 //   simulate <first_iteration> <steps> <tpcd> <min_altitude_km>   -> "stats ..." and "log ..."
+//   conjlog <path>          ConjunctionLogger CSV of the last simulate's log
+//   dataset <path> <c_D>    DatasetReader::readDataset (no device needed)
+//                           -> "data id cospar activity mass radius bc_inv rx ry rz vx vy vz"
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -51,12 +54,27 @@ int main(int argc, char **argv) {
   std::unique_ptr<Container> container;
   std::unique_ptr<Acceleration::AccelerationAccumulator> accumulator;
   std::unique_ptr<YoshidaIntegrator> integrator;
+  std::vector<ladds_logged_pair> last_log;
   std::string line;
   try {
     while (std::getline(in, line)) {
       std::istringstream ls(line);
       std::string cmd;
       if (!(ls >> cmd) || cmd[0] == '#') continue;
+      if (cmd == "dataset") {
+        std::string path;
+        double cd = 0.0;
+        ls >> path >> cd;
+        if (!parsed(ls, line)) return 2;
+        for (const Particle &p : DatasetReader::readDataset(path, cd)) {
+          const auto &r = p.getPosition();
+          const auto &v = p.getVelocity();
+          std::printf("data %zu %s %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", p.getID(),
+                      p.getIdentifier().c_str(), static_cast<int>(p.getActivityState()), p.getMass(), p.getRadius(),
+                      p.getBcInv(), r[0], r[1], r[2], v[0], v[1], v[2]);
+        }
+        continue;
+      }
       if (cmd != "container" && !container) {
         std::fprintf(stderr, "%s before container\n", cmd.c_str());
         return 2;
@@ -114,6 +132,21 @@ int main(int argc, char **argv) {
           std::printf("state %zu %.17g %.17g %.17g %.17g %.17g %.17g\n", it->getID(), r[0], r[1], r[2], v[0], v[1],
                       v[2]);
         }
+      } else if (cmd == "delete") {  // autopas.deleteParticle, as BreakupWrapper.cpp:24-25 does
+        unsigned long long id = 0;
+        ls >> id;
+        if (!parsed(ls, line)) return 2;
+        bool found = false;
+        for (auto it = container->cbegin(); it != container->cend(); ++it)
+          if (it->getID() == id) {
+            container->deleteParticle(*it);
+            found = true;
+            break;
+          }
+        if (!found) {
+          std::fprintf(stderr, "no particle with id %llu\n", id);
+          return 2;
+        }
       } else if (cmd == "burnups") {
         double h = 0.0;
         ls >> h;
@@ -128,9 +161,16 @@ int main(int argc, char **argv) {
         const ladds_run_stats st = container->simulate(first, steps, tpcd, h);
         std::printf("stats %llu %llu %llu\n", (unsigned long long)st.collisions, (unsigned long long)st.evasions,
                     (unsigned long long)st.remaining);
-        for (const ladds_logged_pair &lp : container->fetchLog())
+        last_log = container->fetchLog();
+        for (const ladds_logged_pair &lp : last_log)
           std::printf("log %llu %s %llu %llu %.17g\n", (unsigned long long)lp.iteration, lp.evaded ? "evaded" : "hit",
                       (unsigned long long)lp.pair.id_a, (unsigned long long)lp.pair.id_b, lp.pair.d2);
+      } else if (cmd == "conjlog") {
+        std::string path;
+        ls >> path;
+        if (!parsed(ls, line)) return 2;
+        ConjunctionLogger logger(path);
+        logger.writeLog(last_log);
       } else {
         std::fprintf(stderr, "unknown command: %s\n", cmd.c_str());
         return 2;

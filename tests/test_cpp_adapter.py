@@ -180,3 +180,64 @@ def test_simulate_and_burnups_match_python_binding(tmp_path):
     assert sorted(out["state"]) == sorted(int(i) for i in ids)
     for k, i in enumerate(ids):
         assert np.array_equal(out["state"][int(i)], np.concatenate([r[k], v[k]]))
+
+
+def test_breakup_and_conjunction_log_match_python_binding(tmp_path):
+    """BreakupWrapper.cpp:13-45 through the adapter (deleteParticle of a colliding pair,
+    addParticle of fragments whose parent is the first object), then simulate() and the
+    ConjunctionLogger CSV: same state, pairs and CSV rows as the Python binding."""
+    rng = np.random.default_rng(3)
+    pop = la.generate_population(0, 0x1ADD5 + 9, 1024, planted_pairs=16)
+    cfg = la.EngineConfig(dt=10.0, components=la.KEP | la.J2, cutoff_km=10.0, collision_distance_factor=2.0e5)
+    with la.Engine(cfg) as eng:
+        p = pop.copy()
+        p.aom = np.pi * pop.radius_m * pop.radius_m * 1e-6 / pop.mass_kg
+        eng.set_objects(p)
+        eng.propagate(2)
+        hits, _ = eng.conjunctions(10.0)
+        h = ol.sort_pairs(hits)[0]
+        ida, idb = int(h["id_a"]), int(h["id_b"])
+        r, v = eng.state()
+        ia = int(h["idx_a"])
+        nf = 12
+        fr = np.asarray(h["mid"]) + rng.uniform(-0.3, 0.3, (nf, 3))
+        fv = v[ia] + rng.normal(0.0, 0.02, (nf, 3))
+        frad = rng.uniform(0.05, 0.5, nf)
+        fid = np.arange(5000, 5000 + nf, dtype=np.uint64)
+        assert eng.delete_objects([ida, idb]) == 2
+        eng.add_objects(rx=fr[:, 0], ry=fr[:, 1], rz=fr[:, 2], vx=fv[:, 0], vy=fv[:, 1], vz=fv[:, 2],
+                        radius_m=frad, aom=np.zeros(nf), bc_inv=np.zeros(nf), id=fid,
+                        parent=np.full(nf, ida, np.uint64))
+        after, _ = eng.conjunctions(10.0)
+        st = eng.simulate(3, first_iteration=2, tpcd=1, min_altitude_km=150.0)
+        log = eng.fetch_log()
+        rs, vs = eng.state()
+        ids = eng.ids()
+        la.write_conjunctions_csv(tmp_path / "py.csv", log)
+    lines = _header(10.0, 10.0, comps=(1, 1, 0, 0, 0, 0, 0, 0))
+    for i in range(pop.n):
+        lines.append(_particle((pop.rx[i], pop.ry[i], pop.rz[i]), (pop.vx[i], pop.vy[i], pop.vz[i]), i,
+                               mass=pop.mass_kg[i], radius=pop.radius_m[i], bc=pop.bc_inv[i], cr=pop.cr[i]))
+    lines += ["integrate 2", "functor 10.0 10.0 200000.0 0.05 0.1", f"delete {ida}", f"delete {idb}"]
+    for k in range(nf):
+        lines.append(_particle(fr[k], fv[k], fid[k], radius=frad[k], parent=ida))
+    lines += ["functor 10.0 10.0 200000.0 0.05 0.1", "simulate 2 3 1 150.0", f"conjlog {tmp_path / 'cpp.csv'}",
+              "state"]
+    out = _run(tmp_path, lines)
+    first = sorted((int(x["id_a"]), int(x["id_b"])) for x in hits)
+    second = sorted((int(x["id_a"]), int(x["id_b"])) for x in after)
+    n_first = len(first)
+    assert sorted((a, b) for a, b, _, _ in out["hit"][:n_first]) == first
+    assert sorted((a, b) for a, b, _, _ in out["hit"][n_first:]) == second
+    assert (ida, idb) in first and (ida, idb) not in second
+    assert out["stats"] == [(st["collisions"], st["evasions"], st["remaining"])]
+    assert sorted(out["state"]) == sorted(int(i) for i in ids)
+    for k, i in enumerate(ids):
+        assert np.array_equal(out["state"][int(i)], np.concatenate([rs[k], vs[k]]))
+
+    def rows(path):
+        text = path.read_text().splitlines()
+        assert text[0] == "Iteration,P1,P2,SquaredDistance"
+        return sorted((int(a), int(b), int(c), float(d)) for a, b, c, d in (ln.split(",") for ln in text[1:]))
+
+    assert rows(tmp_path / "cpp.csv") == rows(tmp_path / "py.csv") and len(rows(tmp_path / "py.csv")) > 0
