@@ -138,6 +138,13 @@ typedef struct ladds_logged_pair {
   uint32_t scenario; /* ensemble scenario (0 without scenarios) */
 } ladds_logged_pair; /* 72 bytes */
 
+/* HDF5Definitions::CollisionInfo / EvasionInfo (src/ladds/io/hdf5/HDF5Definitions.h:41-64):
+ * the record HDF5Writer::writeConjunctions stores per pair. */
+typedef struct {
+  uint32_t id_a, id_b;
+  float d2;
+} ladds_pair_record; /* 12 bytes */
+
 typedef struct ladds_run_stats {
   uint64_t collisions;      /* total collisions found (also beyond log capacity) */
   uint64_t evasions;        /* total evaded conjunctions                          */
@@ -221,6 +228,18 @@ LADDS_API int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nstep
                    ladds_run_stats *stats);
 LADDS_API int ladds_fetch_log(ladds_ctx *ctx, ladds_logged_pair *out, size_t cap, size_t *n);
 LADDS_API int ladds_clear_log(ladds_ctx *ctx);
+
+/* Output records, packed on the device so that only narrow data crosses PCIe.
+ * ladds_particle_records: the payload of HDF5Writer::writeParticles
+ * (io/hdf5/HDF5Writer.cpp:89-149): float32 xyz positions and velocities (3 * n floats
+ * each, object order) and uint32 ids of the current objects.
+ * ladds_log_records: the payload of HDF5Writer::writeConjunctions / writeEvasions
+ * (HDF5Writer.cpp:151-187) for one iteration of the device log, unordered; on overflow
+ * LADDS_E_OVERFLOW with *n / *ne = required counts.  Casts as the reference's
+ * static_casts: double -> float to nearest, ids keep their low 32 bits. */
+LADDS_API int ladds_particle_records(ladds_ctx *ctx, float *pos, float *vel, uint32_t *ids);
+LADDS_API int ladds_log_records(ladds_ctx *ctx, uint64_t iteration, ladds_pair_record *hits, size_t cap, size_t *n,
+                                ladds_pair_record *evaded, size_t ecap, size_t *ne);
 
 /* Simulation::deleteBurnUps (Simulation.cpp:561-596): remove objects with |r| < R_E + h_min,
  * compacting the arrays on the device (order preserved). */

@@ -187,6 +187,28 @@ class Container {
     check(ladds_clear_log(ctx_), ctx_, "ladds_clear_log");
     return out;
   }
+  /* What HDF5Writer::writeParticles stores (io/hdf5/HDF5Writer.cpp:89-149), packed on the
+   * device: float32 xyz positions / velocities (3 per object) and uint32 ids. */
+  void particleRecords(std::vector<float> &pos, std::vector<float> &vel, std::vector<uint32_t> &ids) {
+    syncDevice();
+    const size_t n = ladds_num_objects(ctx_);
+    pos.resize(3 * n);
+    vel.resize(3 * n);
+    ids.resize(n);
+    check(ladds_particle_records(ctx_, pos.data(), vel.data(), ids.data()), ctx_, "ladds_particle_records");
+  }
+  /* What HDF5Writer::writeConjunctions / writeEvasions store for one iteration
+   * (HDF5Writer.cpp:151-187), from the device log (call before fetchLog, which clears it). */
+  void logRecords(uint64_t iteration, std::vector<ladds_pair_record> &collisions,
+                  std::vector<ladds_pair_record> &evasions) {
+    size_t n = 0, ne = 0;
+    int rc = ladds_log_records(ctx_, iteration, nullptr, 0, &n, nullptr, 0, &ne);
+    if (rc != LADDS_OK && rc != LADDS_E_OVERFLOW) check(rc, ctx_, "ladds_log_records");
+    collisions.resize(n);
+    evasions.resize(ne);
+    check(ladds_log_records(ctx_, iteration, collisions.data(), n, &n, evasions.data(), ne, &ne), ctx_,
+          "ladds_log_records");
+  }
 
   /* ---- used by the adapter classes ---- */
   ladds_ctx *context() { return ctx_; }

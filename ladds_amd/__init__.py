@@ -26,7 +26,7 @@ import numpy as np
 from . import _abi
 from ._abi import (ALL_COMPONENTS, C22, DRAG, EVASIVE, EVASIVE_PRESERVING, FP64, J2, KEP, LUN, PAIRS_DIGEST,
                    PAIRS_LIST, PASSIVE, S22, SOL, SPLIT, SPLIT_ALL32, SRP, THRESHOLD_ABSOLUTE,
-                   THRESHOLD_RADIUS_SCALED, LOGGED_DTYPE, PAIR_DTYPE)
+                   THRESHOLD_RADIUS_SCALED, LOGGED_DTYPE, PAIR_DTYPE, RECORD_DTYPE)
 
 __all__ = [
     "Engine", "EngineConfig", "LaddsError", "generate_population", "generate_ensemble", "read_dataset",
@@ -34,7 +34,7 @@ __all__ = [
     "components_mask", "KEP", "J2", "C22", "S22", "SOL", "LUN", "SRP", "DRAG", "ALL_COMPONENTS", "FP64", "SPLIT",
     "SPLIT_ALL32",
     "THRESHOLD_RADIUS_SCALED", "THRESHOLD_ABSOLUTE", "PASSIVE", "EVASIVE", "EVASIVE_PRESERVING",
-    "PAIRS_LIST", "PAIRS_DIGEST", "PAIR_DTYPE", "LOGGED_DTYPE",
+    "PAIRS_LIST", "PAIRS_DIGEST", "PAIR_DTYPE", "LOGGED_DTYPE", "RECORD_DTYPE", "population_from_records",
 ]
 
 lib = _abi.lib
@@ -182,6 +182,29 @@ def read_dataset(path, coefficient_of_drag: float = 2.2):
     pop.id = np.array([int(r[0]) for r in rows], np.uint64)
     pop.parent = np.full(n, np.iinfo(np.uint64).max, np.uint64)
     return pop, [r[1] for r in rows]
+
+
+def population_from_records(pos, vel, ids, constant_properties) -> Population:
+    """Checkpoint reload (``HDF5Reader::readParticles``, io/hdf5/HDF5Reader.cpp:25-65): float32
+    state and uint32 ids of one output iteration plus the ConstantProperties table (a mapping
+    or structured array with id, mass, radius, bc_inv, activity), widened to float64.
+    Parents are reset to "none", as the reference does."""
+    table = {k: np.asarray(constant_properties[k]) for k in ("id", "mass", "radius", "bc_inv", "activity")}
+    where = {int(i): k for k, i in enumerate(table["id"])}
+    rows = np.array([where[int(i)] for i in np.asarray(ids)], np.int64)
+    pos = np.asarray(pos, np.float32).astype(np.float64)
+    vel = np.asarray(vel, np.float32).astype(np.float64)
+    mass = table["mass"][rows].astype(np.float32).astype(np.float64)
+    radius = table["radius"][rows].astype(np.float32).astype(np.float64)
+    n = pos.shape[0]
+    pop = Population(rx=pos[:, 0].copy(), ry=pos[:, 1].copy(), rz=pos[:, 2].copy(), vx=vel[:, 0].copy(),
+                     vy=vel[:, 1].copy(), vz=vel[:, 2].copy(), radius_m=radius, mass_kg=mass,
+                     aom=np.pi * radius * radius * 1e-6 / mass,
+                     bc_inv=table["bc_inv"][rows].astype(np.float32).astype(np.float64), cr=np.ones(n))
+    pop.activity = table["activity"][rows].astype(np.int32)
+    pop.id = np.asarray(ids).astype(np.uint64)
+    pop.parent = np.full(n, np.iinfo(np.uint64).max, np.uint64)
+    return pop
 
 
 def write_conjunctions_csv(path, log):
@@ -362,6 +385,30 @@ class Engine:
 
     def clear_log(self):
         self._check(lib.ladds_clear_log(self._h), "clear_log")
+
+    def particle_records(self):
+        """Payload of ``HDF5Writer::writeParticles`` (io/hdf5/HDF5Writer.cpp:89-149), packed on
+        the device: float32 (n,3) positions and velocities and uint32 ids."""
+        n = self.n
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        ids = np.zeros(n, np.uint32)
+        self._check(lib.ladds_particle_records(self._h, pos.ctypes.data, vel.ctypes.data, ids.ctypes.data),
+                    "particle_records")
+        return pos, vel, ids
+
+    def log_records(self, iteration: int):
+        """Payload of ``HDF5Writer::writeConjunctions`` / ``writeEvasions`` (HDF5Writer.cpp:151-187)
+        for one iteration of the device log: two RECORD_DTYPE arrays, unordered."""
+        n, ne = C.c_size_t(0), C.c_size_t(0)
+        rc = lib.ladds_log_records(self._h, int(iteration), None, 0, C.byref(n), None, 0, C.byref(ne))
+        if rc not in (0, _abi.E_OVERFLOW):
+            self._check(rc, "log_records")
+        hits = np.zeros(max(n.value, 1), RECORD_DTYPE)
+        ev = np.zeros(max(ne.value, 1), RECORD_DTYPE)
+        self._check(lib.ladds_log_records(self._h, int(iteration), hits.ctypes.data, hits.shape[0], C.byref(n),
+                                          ev.ctypes.data, ev.shape[0], C.byref(ne)), "log_records")
+        return hits[: n.value], ev[: ne.value]
 
     def delete_burnups(self, min_altitude_km: float) -> int:
         removed = C.c_size_t(0)

@@ -133,7 +133,8 @@ try:
     PAIR_DTYPE = np.dtype([("id_a", "<u8"), ("id_b", "<u8"), ("idx_a", "<u4"), ("idx_b", "<u4"),
                            ("d2", "<f8"), ("mid", "<f8", (3,))])
     LOGGED_DTYPE = np.dtype([("pair", PAIR_DTYPE), ("iteration", "<u8"), ("evaded", "<u4"), ("scenario", "<u4")])
-    assert PAIR_DTYPE.itemsize == 56 and LOGGED_DTYPE.itemsize == 72
+    RECORD_DTYPE = np.dtype([("id_a", "<u4"), ("id_b", "<u4"), ("d2", "<f4")])
+    assert PAIR_DTYPE.itemsize == 56 and LOGGED_DTYPE.itemsize == 72 and RECORD_DTYPE.itemsize == 12
 except ImportError:  # pragma: no cover
     np = None
 
@@ -150,6 +151,9 @@ _SIGNATURES = {
     "ladds_set_objects": (C.c_int, [_ctx, C.POINTER(Objects)]),
     "ladds_num_objects": (C.c_size_t, [_ctx]),
     "ladds_add_objects": (C.c_int, [_ctx, C.POINTER(Objects)]),
+    "ladds_particle_records": (C.c_int, [_ctx, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ladds_log_records": (C.c_int, [_ctx, C.c_uint64, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), C.c_void_p,
+                                    C.c_size_t, C.POINTER(C.c_size_t)]),
     "ladds_delete_objects": (C.c_int, [_ctx, C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(C.c_size_t)]),
     "ladds_set_scenarios": (C.c_int, [_ctx, C.c_size_t, C.POINTER(C.c_uint64)]),
     "ladds_num_scenarios": (C.c_size_t, [_ctx]),

@@ -30,6 +30,7 @@ SOURCES = [
     "ephemeris.cpp",
     "population.cpp",
     "propagate.hip",
+    "export.hip",
     "conjunction.hip",
     "compact.hip",
     "tiles.hip",

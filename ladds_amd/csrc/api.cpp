@@ -128,6 +128,11 @@ struct ladds_ctx {
   void *scratch2 = nullptr;
   size_t scratch2_bytes = 0;
 
+  // Staging for the output records (ladds_particle_records, ladds_log_records).
+  void *xbuf = nullptr;
+  size_t xbuf_bytes = 0;
+  unsigned long long *xcount = nullptr;
+
   uint64_t step_count = 0;  // steps since t0
   double c[4], dcoef[3];
 
@@ -819,6 +824,8 @@ int ladds_destroy(ladds_ctx *ctx) {
   hipFree(ctx->log);
   hipFree(ctx->log_count);
   hipFree(ctx->counters);
+  hipFree(ctx->xbuf);
+  hipFree(ctx->xcount);
   for (auto &kv : ctx->prof)
     for (EventPair &ev : kv.second.used) ctx->event_pool.push_back(ev);
   for (EventPair &ev : ctx->event_pool) {
@@ -1475,6 +1482,71 @@ int ladds_fetch_log(ladds_ctx *ctx, ladds_logged_pair *out, size_t cap, size_t *
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (cnt > ctx->log_cap) return ctx->fail(LADDS_E_OVERFLOW, "device pair log overflowed (log_capacity)");
   if (cnt > cap) return ctx->fail(LADDS_E_OVERFLOW, "output capacity too small");
+  return LADDS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int ensure_xbuf(ladds_ctx *ctx, size_t bytes) {
+  if (!ctx->xcount) HIPCHK(ctx, dalloc(&ctx->xcount, 2));
+  if (bytes <= ctx->xbuf_bytes) return LADDS_OK;
+  hipFree(ctx->xbuf);
+  ctx->xbuf = nullptr;
+  ctx->xbuf_bytes = 0;
+  HIPCHK(ctx, hipMalloc(&ctx->xbuf, bytes));
+  ctx->xbuf_bytes = bytes;
+  return LADDS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ladds_particle_records(ladds_ctx *ctx, float *pos, float *vel, uint32_t *ids) {
+  if (!ctx) return LADDS_E_INVALID;
+  const size_t n = ctx->n;
+  if (n && (!pos || !vel || !ids)) return LADDS_E_INVALID;
+  if (n == 0) return LADDS_OK;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t fb = sizeof(float) * 3 * n;
+  int rc = ensure_xbuf(ctx, 2 * fb + sizeof(uint32_t) * n);
+  if (rc != LADDS_OK) return rc;
+  float *dpos = static_cast<float *>(ctx->xbuf), *dvel = dpos + 3 * n;
+  uint32_t *dids = reinterpret_cast<uint32_t *>(dvel + 3 * n);
+  HIPCHK(ctx, timed(ctx, "particle_records",
+                    [&] { return launch_particle_records(ctx->d, n, dpos, dvel, dids, ctx->stream); }));
+  HIPCHK(ctx, hipMemcpyAsync(pos, dpos, fb, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(vel, dvel, fb, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ids, dids, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return LADDS_OK;
+}
+
+int ladds_log_records(ladds_ctx *ctx, uint64_t iteration, ladds_pair_record *hits, size_t cap, size_t *n,
+                      ladds_pair_record *evaded, size_t ecap, size_t *ne) {
+  if (!ctx || !n || !ne || (cap && !hits) || (ecap && !evaded)) return LADDS_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t rb = sizeof(PairRecord32);
+  int rc = ensure_xbuf(ctx, std::max<size_t>(rb * (cap + ecap), 64));
+  if (rc != LADDS_OK) return rc;
+  PairRecord32 *dh = static_cast<PairRecord32 *>(ctx->xbuf), *de = dh + cap;
+  HIPCHK(ctx, hipMemsetAsync(ctx->xcount, 0, 2 * sizeof(unsigned long long), ctx->stream));
+  HIPCHK(ctx, timed(ctx, "log_records", [&] {
+    return launch_log_records(ctx->log, ctx->log_count, ctx->log_cap, iteration, dh, cap, de, ecap, ctx->xcount,
+                              ctx->stream);
+  }));
+  unsigned long long cnt[2] = {0, 0};
+  HIPCHK(ctx, hipMemcpyAsync(cnt, ctx->xcount, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *n = (size_t)cnt[0];
+  *ne = (size_t)cnt[1];
+  const size_t mh = std::min<size_t>(cnt[0], cap), me = std::min<size_t>(cnt[1], ecap);
+  if (mh) HIPCHK(ctx, hipMemcpyAsync(hits, dh, rb * mh, hipMemcpyDeviceToHost, ctx->stream));
+  if (me) HIPCHK(ctx, hipMemcpyAsync(evaded, de, rb * me, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (cnt[0] > cap || cnt[1] > ecap) return ctx->fail(LADDS_E_OVERFLOW, "output capacity too small");
   return LADDS_OK;
 }
 

@@ -104,6 +104,11 @@ struct PairRecord {  // same layout as ladds_pair
   double mid[3];
 };
 
+struct PairRecord32 {  // same layout as ladds_pair_record
+  uint32_t id_a, id_b;
+  float d2;
+};
+
 struct LoggedPair {  // same layout as ladds_logged_pair
   PairRecord pair;
   uint64_t iteration;

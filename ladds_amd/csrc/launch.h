@@ -59,4 +59,12 @@ hipError_t launch_compact(const DeviceObjects &src, const DeviceObjects &dst, co
 
 constexpr uint32_t kCompactTile = 2048;
 
+// Output records (export.hip): float32 AoS state + uint32 ids; one iteration's pairs as
+// {uint32, uint32, float32}.  counts[2] must be zero before launch_log_records.
+hipError_t launch_particle_records(const DeviceObjects &o, size_t n, float *pos, float *vel, uint32_t *ids,
+                                   hipStream_t s);
+hipError_t launch_log_records(const LoggedPair *log, const unsigned long long *log_count, uint64_t log_cap,
+                              uint64_t iteration, PairRecord32 *hits, uint64_t hcap, PairRecord32 *ev, uint64_t ecap,
+                              unsigned long long *counts, hipStream_t s);
+
 }  // namespace ladds
