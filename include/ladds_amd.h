@@ -67,9 +67,9 @@ extern "C" {
 
 /* ---- precision forms ---- */
 #define LADDS_FP64 0       /* every stage in FP64                                                   */
-#define LADDS_SPLIT 1      /* FP64: state, Kepler, J2, closest approach; FP32: C22/S22, Sun, Moon,  */
-                           /* SRP, drag and the conservative conjunction prefilter                  */
-#define LADDS_SPLIT_ALL32 2 /* as LADDS_SPLIT with J2 also in FP32 (larger position error)          */
+#define LADDS_SPLIT 1      /* FP64: state, Kepler, J2, C22/S22, drag, closest approach;              */
+                           /* FP32: Sun, Moon, SRP and the conservative conjunction prefilter       */
+#define LADDS_SPLIT_ALL32 2 /* as LADDS_SPLIT with J2 and C22/S22 also in FP32 (larger error)       */
 
 /* ---- conjunction threshold modes ---- */
 #define LADDS_THRESHOLD_RADIUS_SCALED 0 /* reference: d^2 <= ((R_i+R_j) cdf / 1000)^2, CollisionFunctor.cpp:85,112 */

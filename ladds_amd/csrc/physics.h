@@ -14,15 +14,18 @@
 // Precision forms:
 //   kFormFp64      every term in FP64.  Third-body and SRP use the direct form
 //                  GM (d/|d|^3 - s/|s|^3): its cancellation costs ~1e-12 relative in FP64.
-//   kFormSplit     Kepler, J2 and drag in FP64; C22, S22, Sun, Moon and SRP in FP32,
+//   kFormSplit     Kepler, J2, C22/S22 and drag in FP64; Sun, Moon and SRP in FP32,
 //                  accumulated into the FP64 kick.  J2's acceleration needs ~29
 //                  significant bits to hold 1 cm/day (FP32 has 24), and drag moves a
 //                  low object hundreds of km along-track per day, so FP32 rounding of
 //                  either costs centimetres (measured: up to 1 m for J2, 25 cm for drag).
-//                  The FP32 third-body terms use Battin's cancellation-free form
-//                  a = -GM/|s|^3 (1+q)^-3/2 (r + f(q) s) (Battin 1987, eq. 8.61).
+//                  The tesseral terms in FP32 (Earth-fixed rotation and field) leave
+//                  0.5 cm at the 99.9th percentile and over 1 cm in the tail of a
+//                  1M-object day, so they stay FP64 too; the FP32 terms then contribute
+//                  ~0.01 cm.  The FP32 third-body terms use Battin's cancellation-free
+//                  form a = -GM/|s|^3 (1+q)^-3/2 (r + f(q) s) (Battin 1987, eq. 8.61).
 //                  The SRP shadow decision is taken on the FP64 state.
-//   kFormSplitAll32  as kFormSplit with J2 also in FP32.
+//   kFormSplitAll32  J2, C22/S22, Sun, Moon and SRP in FP32 (Kepler and drag FP64).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -212,7 +215,11 @@ __device__ __forceinline__ void acceleration(double x, double y, double z, doubl
     if (m & kLunBit) third_body64(x, y, z, e64.moon, kGmMoon, e64.moon_acc, ax, ay, az);
     if ((m & kSrpBit) && !in_shadow(x, y, z, r2, e64)) srp64(x, y, z, e64.sun, p.srp64, ax, ay, az);
   } else {
-    if (m & ~(kKepBit | kDragBit | (j2_64 ? kJ2Bit : 0u))) {
+    constexpr bool tess_64 = (FORM == kFormSplit);
+    if (tess_64 && (m & (kC22Bit | kS22Bit)))
+      tesseral<double>(x, y, z, inv_r2, inv_r3 * inv_r2, m, e64.ct, e64.st, ax, ay, az);
+    constexpr unsigned kFp64Bits = kKepBit | kDragBit | (j2_64 ? kJ2Bit : 0u) | (tess_64 ? (kC22Bit | kS22Bit) : 0u);
+    if (m & ~kFp64Bits) {
       const float xf = (float)x, yf = (float)y, zf = (float)z;
       const float r2f = (float)r2;
       const float inv_r2f = (float)inv_r2;
@@ -225,7 +232,7 @@ __device__ __forceinline__ void acceleration(double x, double y, double z, doubl
         fy += f * yf * (1.f - z2);
         fz += f * zf * (3.f - z2);
       }
-      if (m & (kC22Bit | kS22Bit))
+      if (!tess_64 && (m & (kC22Bit | kS22Bit)))
         tesseral<float>(xf, yf, zf, inv_r2f, inv_r3f * inv_r2f, m, e32.ct, e32.st, fx, fy, fz);
       const bool sun_grav = (m & kSolBit) != 0;
       const float srp = ((m & kSrpBit) && !in_shadow(x, y, z, r2, e64)) ? p.srp32 : 0.f;
