@@ -252,9 +252,10 @@ LADDS_API int ladds_delete_burnups(ladds_ctx *ctx, double min_altitude_km, size_
  * candidate workgroups per plane (1-32, for planes dense with clusters), pair-kernel
  * workgroup sizes, cell sizes (1, 1.25, 1.5, 2 x the search radius), plane widths (1, 1.5,
  * 2, 3 x the cell size), plane shapes (log2 buckets per x-plane around the
- * population-derived default) and partition tile sizes
- * (4096, 8192, 16384 objects), one dimension at a time (coordinate descent), and keep the
- * fastest for this context.  Pair results do not depend on the choice. */
+ * population-derived default), partition tile sizes (4096, 8192, 16384 objects) and the
+ * pair kernel's LDS per plane pair (sized for 2-5 workgroups per CU, or automatic), one
+ * dimension at a time (coordinate descent), and keep the fastest for this context.  Pair
+ * results do not depend on the choice. */
 typedef struct ladds_tuning {
   double cell_size_km;   /* chosen cell side [km]                               */
   int32_t plane_bits;    /* chosen log2 buckets per x-plane                     */
@@ -264,7 +265,7 @@ typedef struct ladds_tuning {
   int32_t tried;         /* configurations timed                                */
   int32_t tile_size;     /* chosen objects per partition tile                   */
   int32_t chunks;        /* chosen workgroups per plane (dense planes)          */
-  int32_t reserved;
+  int32_t workgroups_per_cu; /* pair-kernel workgroups per CU its LDS is sized for */
   double plane_width_km; /* chosen cell length along x (plane width) [km]       */
 } ladds_tuning;
 

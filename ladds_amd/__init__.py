@@ -442,8 +442,10 @@ class Engine:
         t = _abi.Tuning()
         dt = self.config.dt if dt_interp is None else dt_interp
         self._check(lib.ladds_autotune(self._h, float(dt), int(repetitions), C.byref(t)), "autotune")
-        return {"cell_size_km": t.cell_size_km, "plane_width_km": t.plane_width_km, "plane_bits": t.plane_bits, "pair_threads": t.pair_threads,
-                "tile_size": t.tile_size, "chunks": t.chunks, "scan_us": t.scan_us, "default_us": t.default_us, "tried": t.tried}
+        return {"cell_size_km": t.cell_size_km, "plane_width_km": t.plane_width_km, "plane_bits": t.plane_bits,
+                "pair_threads": t.pair_threads, "tile_size": t.tile_size, "chunks": t.chunks,
+                "workgroups_per_cu": t.workgroups_per_cu, "scan_us": t.scan_us, "default_us": t.default_us,
+                "tried": t.tried}
 
     # -- measurement ----------------------------------------------------
     def set_profiling(self, enable: bool):

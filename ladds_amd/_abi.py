@@ -107,7 +107,7 @@ class Tuning(C.Structure):
         ("tried", C.c_int32),
         ("tile_size", C.c_int32),
         ("chunks", C.c_int32),
-        ("reserved", C.c_int32),
+        ("workgroups_per_cu", C.c_int32),
         ("plane_width_km", C.c_double),
     ]
 

@@ -109,6 +109,7 @@ struct ladds_ctx {
   int tune_tile = 0;
   int tune_chunks = 0;
   double tune_fx = 0.0;  // plane width / cell size
+  int tune_wgs = 0;      // pair-kernel workgroups per CU its LDS is sized for (0: automatic)
 
   StepState *d_step = nullptr;  // graph replay position (ladds_config.step_graph)
 
@@ -383,9 +384,22 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   tp.nchunk = (uint32_t)std::min<size_t>(16, std::max<size_t>(1, own_max / (4 * (size_t)tp.pair_threads)));
   if (ctx->cfg.scan_chunks >= 1 && ctx->cfg.scan_chunks <= 32) tp.nchunk = (uint32_t)ctx->cfg.scan_chunks;
   if (ctx->tune_chunks >= 1 && ctx->tune_chunks <= 32) tp.nchunk = (uint32_t)ctx->tune_chunks;
-  const size_t want = std::max<size_t>(1024, ((pair_max + pair_max / 4) + 255) / 256 * 256);
-  const size_t fit = (kPairTileLdsBytes - 4 * (size_t)tp.lds_header_words) / rec;
-  tp.cap_items = (uint32_t)std::min(want, fit);
+  // LDS of a plane pair: room for the fullest plane pair now, then grown to the whole LDS
+  // share of as many workgroups per CU as that size allows (slack for the planes' drift at
+  // no cost in occupancy, which bounds the latency-bound scan).  Plane pairs beyond the
+  // capacity are sorted in the global scratch arena.
+  {
+    const size_t hdr = 4 * (size_t)tp.lds_header_words;
+    const size_t fit = (kPairTileLdsBytes - hdr) / rec;
+    const size_t need = std::min(fit, std::max<size_t>(1024, (pair_max + 63) / 64 * 64));
+    const size_t wave_cap = std::max<size_t>(1, 16 / std::max<uint32_t>(1, tp.pair_threads / 64));
+    size_t wgs = std::min(wave_cap, kPairTileLdsBytes / (hdr + need * rec + kLdsMargin));
+    if (ctx->tune_wgs >= 1 && ctx->tune_wgs <= 8) wgs = (size_t)ctx->tune_wgs;
+    wgs = std::max<size_t>(1, wgs);
+    const size_t share = kPairTileLdsBytes / wgs;
+    const size_t cap = share > hdr + kLdsMargin ? (share - hdr - kLdsMargin) / rec / 64 * 64 : 0;
+    tp.cap_items = (uint32_t)std::min(fit, std::max<size_t>(cap, 256));
+  }
   // scratch arena for planes beyond the LDS capacity: every chunk of a plane sorts it
   const size_t arena_need = std::max<size_t>(2 * n, 1024) * tp.nchunk;
   const size_t arena_bytes = arena_need * sizeof(RecordF64);
@@ -1165,13 +1179,14 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   ctx->tune_tile = 0;
   ctx->tune_chunks = 0;
   ctx->tune_fx = 0.0;
+  ctx->tune_wgs = 0;
   int rc = setup_grid(ctx, n, rx.data());
   if (rc != LADDS_OK) return rc;
   const double base_cell = 1.0 / ctx->grid.inv_cell;
   const int base_pb = (int)(ctx->grid.by + ctx->grid.bz);
   // current point of the search: {cell, plane bits, threads, tile, chunks}
   double cur_cell = base_cell, cur_fx = 1.0;
-  int cur[4] = {base_pb, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk};
+  int cur[5] = {base_pb, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk, 0};
   double default_us = 0.0;
   rc = time_detection(ctx, dt_interp, repetitions, &default_us);
   if (rc != LADDS_OK) return rc;
@@ -1184,22 +1199,25 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     ctx->tune_threads = v[1];
     ctx->tune_tile = v[2];
     ctx->tune_chunks = v[3];
+    ctx->tune_wgs = v[4];
     return setup_grid(ctx, n, rx.data());
   };
   // Coordinate descent, one dimension at a time, keeping every improvement: chunks per
-  // plane (load balance of dense planes), workgroup size, cell size, plane shape, tile.
+  // plane (load balance of dense planes), workgroup size, cell size, plane shape, tile,
+  // LDS share (workgroups per CU).
   const int chunk_opts[] = {1, 2, 4, 8, 16, 32};
   const int thread_opts[] = {128, 256, 512, 1024};
   const double cell_opts[] = {1.0, 1.25, 1.5, 2.0};
   const int tile_opts[] = {4096, 8192, 16384};
   const double fx_opts[] = {1.0, 1.5, 2.0, 3.0};
+  const int wgs_opts[] = {0, 2, 3, 4, 5};
   struct Cand {
     double cell, fx;
-    std::array<int, 4> v;
+    std::array<int, 5> v;
   };
-  for (int dim = 0; dim < 6; ++dim) {
+  for (int dim = 0; dim < 7; ++dim) {
     std::vector<Cand> cands;
-    std::array<int, 4> c{cur[0], cur[1], cur[2], cur[3]};
+    std::array<int, 5> c{cur[0], cur[1], cur[2], cur[3], cur[4]};
     if (dim == 0)
       for (int k : chunk_opts) { c[3] = k; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 1)
@@ -1213,9 +1231,11 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
         if (base_pb + d >= 4 && base_pb + d <= 12) { c[0] = base_pb + d; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 5)
       for (int t : tile_opts) { c[2] = t; cands.push_back({cur_cell, cur_fx, c}); }
+    if (dim == 6)
+      for (int w : wgs_opts) { c[4] = w; cands.push_back({cur_cell, cur_fx, c}); }
     for (const Cand &cand : cands) {
       if (cand.cell == cur_cell && cand.fx == cur_fx && cand.v[0] == cur[0] && cand.v[1] == cur[1] &&
-          cand.v[2] == cur[2] && cand.v[3] == cur[3])
+          cand.v[2] == cur[2] && cand.v[3] == cur[3] && cand.v[4] == cur[4])
         continue;
       rc = apply(cand.cell, cand.fx, cand.v.data());
       if (rc != LADDS_OK) return rc;
@@ -1227,7 +1247,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
         best_us = us;
         cur_cell = cand.cell;
         cur_fx = cand.fx;
-        for (int k = 0; k < 4; ++k) cur[k] = cand.v[k];
+        for (int k = 0; k < 5; ++k) cur[k] = cand.v[k];
       }
     }
   }
@@ -1243,6 +1263,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     out->pair_threads = (int32_t)ctx->tiles.pair_threads;
     out->tile_size = (int32_t)ctx->tiles.tile_size;
     out->chunks = (int32_t)ctx->tiles.nchunk;
+    out->workgroups_per_cu = (int32_t)ctx->tune_wgs;
     out->scan_us = best_us;
     out->default_us = default_us;
     out->tried = tried;

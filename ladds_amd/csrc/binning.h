@@ -64,4 +64,13 @@ __device__ __forceinline__ CellInfo cell_of(double x, double y, double z, const 
   return c;
 }
 
+// Folded in-plane bucket (the low by + bz bits of cell_of(...).bucket) from y and z only.
+__device__ __forceinline__ uint32_t yz_bucket(double y, double z, const GridParams &g) {
+  uint32_t ly, lz;
+  const int32_t iy = cell_axis(y, g.inv_cell, ly);
+  const int32_t iz = cell_axis(z, g.inv_cell, lz);
+  const uint32_t my = (1u << g.by) - 1u, mz = (1u << g.bz) - 1u;
+  return (((uint32_t)iy & my) << g.bz) | ((uint32_t)iz & mz);
+}
+
 }  // namespace ladds

@@ -42,6 +42,8 @@ hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, cons
 uint32_t pair_tiles_header_words(const TileParams &tp);
 hipError_t configure_tile_kernels();
 constexpr uint32_t kPairTileLdsBytes = 160 * 1024;
+// LDS reserved per pair-kernel workgroup beyond its request (allocation granularity).
+constexpr uint32_t kLdsMargin = 1024;
 constexpr uint32_t kPartitionTile = 8192;
 
 // Search radius of the pair scan: the cutoff in the FP64 form; in the split form the

@@ -208,17 +208,18 @@ __device__ __forceinline__ bool within(const RecOf<FORM> &a, const RecOf<FORM> &
   }
 }
 
-// Bucket ranges of the sorted plane arrays: cur[b] = end of bucket b after placement.
-__device__ __forceinline__ uint32_t bucket_begin(const uint32_t *cur, uint32_t b) { return b ? cur[b - 1] : 0u; }
-
 // Cells have side >= the search radius R, so partners within R lie in the 3x3x3 cells
 // around an object.  Within its own plane a pair is taken from the lower sorted
 // position; pairs with the next plane are all taken here (the next plane's own scan
 // never looks back), pairs with the previous plane by the previous plane's workgroup.
 // Per object: six rows (three y-rows of its own plane, three of the next); in each row
-// the buckets fz-1, fz, fz+1 (mod Kz) are one contiguous run of the sorted array, or two
-// runs where the fold wraps.  The rows are walked by one loop so that the candidate path
-// (the exact predicate and the wavefront compaction) exists once in the code.
+// the buckets fz-1, fz, fz+1 (mod Kz) are one contiguous run of the sorted array plus,
+// where the fold wraps (fz = 0 or Kz-1), a one-bucket run at the other end of the row.
+// Both runs are read unconditionally (the second one empty when there is no wrap), so
+// the lanes of a wavefront take the same path.  cur[b] is the end of bucket b in the
+// sorted array and cur[-1] = 0, so bucket b spans [cur[b-1], cur[b]).  The rows are
+// walked by one loop so that the candidate path (the exact predicate and the wavefront
+// compaction) exists once in the code.
 template <int FORM, bool DIGEST, bool CHUNKED>
 __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FORM> *own, const RecOf<FORM> *halo,
                                            const uint32_t *cur_own, const uint32_t *cur_halo, uint32_t n_own,
@@ -235,8 +236,11 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
     if (CHUNKED && me.idx % nchunk != chunk) continue;
     double x, y, z;
     binning_point_of(me, x, y, z);
-    const uint32_t lb = cell_of(x, y, z, g).bucket;
+    const uint32_t lb = yz_bucket(y, z, g);
     const uint32_t fy = (lb >> sy) & my, fz = lb & mz;
+    const uint32_t zlo = fz ? fz - 1u : 0u, zhi = fz < mz ? fz + 1u : mz;
+    const bool wraps = (fz == 0u) || (fz == mz);
+    const uint32_t zw = fz ? 0u : mz;  // the wrapped bucket when `wraps`
 #pragma unroll 1
     for (uint32_t kind = 0; kind < 6u; ++kind) {
       const bool next_plane = (kind & 1u) != 0u;
@@ -249,21 +253,10 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
       const bool by_index = CHUNKED && !next_plane;
       const uint32_t from = (next_plane || by_index) ? 0u : s + 1u;
       const uint32_t row = ((fy + (kind >> 1) - 1u) & my) << sy;
-      uint32_t lo0, hi0, lo1 = 0u, hi1 = 0u;
-      if (fz != 0u && fz != mz) {
-        lo0 = bucket_begin(cur, row | (fz - 1u));
-        hi0 = cur[row | (fz + 1u)];
-      } else if (fz == 0u) {
-        lo0 = bucket_begin(cur, row);
-        hi0 = cur[row | 1u];
-        lo1 = bucket_begin(cur, row | mz);
-        hi1 = cur[row | mz];
-      } else {
-        lo0 = bucket_begin(cur, row | (mz - 1u));
-        hi0 = cur[row | mz];
-        lo1 = bucket_begin(cur, row);
-        hi1 = cur[row];
-      }
+      const uint32_t b_lo = row | zlo, b_w = row | zw;
+      const uint32_t lo0 = cur[(int32_t)b_lo - 1], hi0 = cur[row | zhi];
+      const uint32_t w0 = cur[(int32_t)b_w - 1], w1 = cur[b_w];
+      const uint32_t lo1 = wraps ? w0 : 0u, hi1 = wraps ? w1 : 0u;
       uint32_t q = max(lo0, from), end = hi0;
 #pragma unroll 1
       for (int part = 0; part < 2; ++part) {
@@ -280,26 +273,27 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
   }
 }
 
+// One work item of the pair scan: plane pair `wid` = (scenario, plane, chunk), sorted in
+// LDS up to tp.cap_items records, beyond that in the global scratch arena.
 template <int FORM, bool DIGEST, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
+__device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds, uint32_t wid, ScanAcc &acc) {
   constexpr int kPairRegItems = kPairRegCapacity / THREADS < 8 ? kPairRegCapacity / THREADS : 8;
   using REC = RecOf<FORM>;
-  extern __shared__ uint32_t lds[];
   const TileParams &tp = a.tp;
   const uint32_t P = tp.nplanes, L = 1u << tp.plane_bits;
-  // workgroup = (scenario, plane, chunk); a scenario's tiles are [t0, t0 + T)
-  const uint32_t chunk = blockIdx.x % tp.nchunk, pidx = blockIdx.x / tp.nchunk;
+  // a scenario's tiles are [t0, t0 + T)
+  const uint32_t chunk = wid % tp.nchunk, pidx = wid / tp.nchunk;
   const uint32_t scen = pidx / P;
   const uint32_t plane = pidx - scen * P, next = (plane + 1u) & (P - 1u);  // P >= 4
   const uint32_t t0 = tp.scen_tile ? tp.scen_tile[scen] : 0u;
   const uint32_t T = tp.scen_tile ? tp.scen_tile[scen + 1] - t0 : tp.ntiles;
-  // LDS layout
-  uint32_t *seg_own = lds;               // T + 1 prefix of segment lengths
-  uint32_t *seg_next = seg_own + (T + 1);
-  uint32_t *src_own = seg_next + (T + 1);  // T source offsets
-  uint32_t *src_next = src_own + T;
-  uint32_t *cur_own = src_next + T;        // L bucket cursors
-  uint32_t *cur_next = cur_own + L;
+  // LDS layout.  Segments: the plane's records in each of the T tiles, then the next
+  // plane's records in each tile; seg[] is their prefix (2T + 1 entries), so record j of
+  // the plane pair (own records first) lies in segment t with seg[t] <= j < seg[t+1].
+  uint32_t *seg = lds;
+  uint32_t *src = seg + (2u * T + 1u);     // 2T source offsets
+  uint32_t *cur_own = src + 2u * T + 1u;   // L bucket cursors, cur_own[-1] = 0
+  uint32_t *cur_next = cur_own + L + 1u;   // L bucket cursors, cur_next[-1] = 0
   uint32_t *wsum = cur_next + L;           // THREADS / 64 + 1
   uint32_t *misc = wsum + (kMaxPairThreads / 64 + 1);  // [0] scratch offset
   REC *lds_items = reinterpret_cast<REC *>(lds + tp.lds_header_words);
@@ -311,27 +305,26 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
     const uint32_t o0 = row[plane], o1 = row[plane + 1];
     const uint32_t n0 = row[next], n1 = row[next + 1];
     const uint32_t tile_base = tp.tile_begin ? tp.tile_begin[g] : g * tp.tile_size;
-    seg_own[t] = o1 - o0;
-    seg_next[t] = n1 - n0;
-    src_own[t] = tile_base + o0;
-    src_next[t] = tile_base + n0;
+    seg[t] = o1 - o0;
+    seg[T + t] = n1 - n0;
+    src[t] = tile_base + o0;
+    src[T + t] = tile_base + n0;
   }
   for (uint32_t b = threadIdx.x; b < L; b += blockDim.x) {
     cur_own[b] = 0;
     cur_next[b] = 0;
   }
-  __syncthreads();
-  const uint32_t n_own = block_exclusive_scan(seg_own, T, wsum);
-  const uint32_t n_next = block_exclusive_scan(seg_next, T, wsum);
   if (threadIdx.x == 0) {
-    seg_own[T] = n_own;
-    seg_next[T] = n_next;
+    cur_own[-1] = 0;
+    cur_next[-1] = 0;
   }
-  ScanAcc acc;
+  __syncthreads();
+  const uint32_t total = block_exclusive_scan(seg, 2u * T, wsum);
+  const uint32_t n_own = seg[T];
+  if (threadIdx.x == 0) seg[2u * T] = total;
   // chunks this plane uses: ~kChunkRecords own records each, at most tp.nchunk
   const uint32_t cp = min(tp.nchunk, max(1u, (n_own + kChunkRecords - 1u) / kChunkRecords));
   if (n_own != 0u && chunk < cp) {
-    const uint32_t total = n_own + n_next;
     const bool in_lds = total <= tp.cap_items;
     if (!in_lds && threadIdx.x == 0) {
       const uint32_t off = atomicAdd(a.arena_top, total);
@@ -340,45 +333,44 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
       if (!misc[1]) atomicAdd(&a.sink.counters->overflow, 1ull);
     }
     __syncthreads();
-    if (!in_lds && !misc[1]) {  // no room: the plane is reported, never written out of bounds
-      flush_counters<DIGEST>(a.sink, acc);
-      return;
-    }
+    if (!in_lds && !misc[1]) return;  // no room: the plane is reported, never written out of bounds
     REC *sorted = in_lds ? lds_items : static_cast<REC *>(a.scratch) + misc[0];
     REC *own = sorted, *halo = sorted + n_own;
-    const uint32_t mloc = L - 1u;
-    // j enumerates own then next records; loc() finds record j in its tile segment
-    auto loc = [&](uint32_t j) -> uint32_t {
-      const bool is_own = j < n_own;
-      const uint32_t k = is_own ? j : j - n_own;
-      const uint32_t *seg = is_own ? seg_own : seg_next;
-      uint32_t lo = 0, hi = T;  // segment t with seg[t] <= k < seg[t+1]
+    // Each thread reads a block of consecutive records: one binary search for the
+    // block's first record, then a forward walk over the segment boundaries.
+    auto find = [&](uint32_t j) -> uint32_t {
+      uint32_t lo = 0, hi = 2u * T;
       while (hi - lo > 1u) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (seg[mid] <= k)
+        if (seg[mid] <= j)
           lo = mid;
         else
           hi = mid;
       }
-      return (is_own ? src_own : src_next)[lo] + (k - seg[lo]);
+      return lo;
     };
     auto local_bucket = [&](const REC &r) -> uint32_t {
       double x, y, z;
       binning_point_of(r, x, y, z);
-      return cell_of(x, y, z, a.g).bucket & mloc;
+      return yz_bucket(y, z, a.g);
     };
     if (total <= (uint32_t)(kPairRegItems * THREADS)) {
       // records stay in registers between the histogram and the placement pass
       REC reg[kPairRegItems];
       uint32_t lb[kPairRegItems];
+      const uint32_t jb = threadIdx.x * (uint32_t)kPairRegItems;
+      uint32_t t = jb < total ? find(jb) : 0u;
 #pragma unroll
       for (int k = 0; k < kPairRegItems; ++k) {
-        const uint32_t j = threadIdx.x + k * THREADS;
-        if (j < total) reg[k] = items[loc(j)];
+        const uint32_t j = jb + k;
+        if (j < total) {
+          while (seg[t + 1u] <= j) ++t;
+          reg[k] = items[src[t] + (j - seg[t])];
+        }
       }
 #pragma unroll
       for (int k = 0; k < kPairRegItems; ++k) {
-        const uint32_t j = threadIdx.x + k * THREADS;
+        const uint32_t j = jb + k;
         if (j < total) {
           lb[k] = local_bucket(reg[k]);
           atomicAdd(&(j < n_own ? cur_own : cur_next)[lb[k]], 1u);
@@ -389,7 +381,7 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
       block_exclusive_scan(cur_next, L, wsum);
 #pragma unroll
       for (int k = 0; k < kPairRegItems; ++k) {
-        const uint32_t j = threadIdx.x + k * THREADS;
+        const uint32_t j = jb + k;
         if (j < total) {
           const bool is_own = j < n_own;
           const uint32_t pos = atomicAdd(&(is_own ? cur_own : cur_next)[lb[k]], 1u);
@@ -399,22 +391,24 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
       __syncthreads();
     } else {
       // pass 1: bucket histogram; pass 2: placement (records read twice)
-      // Records are fetched kBatch at a time per thread so that their locations (LDS
-      // searches) and global loads overlap instead of forming one dependent chain.
       constexpr uint32_t kBatch = 4;
 #pragma unroll 1
       for (int pass = 0; pass < 2; ++pass) {
 #pragma unroll 1
-        for (uint32_t j0 = threadIdx.x; j0 < total; j0 += kBatch * THREADS) {
+        for (uint32_t jb = threadIdx.x * kBatch; jb < total; jb += kBatch * THREADS) {
           REC r[kBatch];
+          uint32_t t = find(jb);
 #pragma unroll
           for (uint32_t u = 0; u < kBatch; ++u) {
-            const uint32_t j = j0 + u * THREADS;
-            if (j < total) r[u] = items[loc(j)];
+            const uint32_t j = jb + u;
+            if (j < total) {
+              while (seg[t + 1u] <= j) ++t;
+              r[u] = items[src[t] + (j - seg[t])];
+            }
           }
 #pragma unroll
           for (uint32_t u = 0; u < kBatch; ++u) {
-            const uint32_t j = j0 + u * THREADS;
+            const uint32_t j = jb + u;
             if (j >= total) continue;
             const bool is_own = j < n_own;
             uint32_t *cur = is_own ? cur_own : cur_next;
@@ -449,6 +443,14 @@ __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
         scan_plane<FORM, DIGEST, false>(a, own, halo, cur_own, cur_next, n_own, THREADS, scen, 0u, 1u, acc);
     }
   }
+}
+
+// One workgroup per plane pair (grid nplanes * nscen * nchunk).
+template <int FORM, bool DIGEST, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
+  extern __shared__ uint32_t lds[];
+  ScanAcc acc;
+  pair_plane<FORM, DIGEST, THREADS>(a, lds, blockIdx.x, acc);
   flush_counters<DIGEST>(a.sink, acc);
 }
 
@@ -543,13 +545,16 @@ hipError_t configure_tile_kernels() {
 uint32_t pair_tiles_header_words(const TileParams &tp) {
   const uint32_t L = 1u << tp.plane_bits;
   const uint32_t T = tp.nscen > 1 ? tp.max_scen_tiles : tp.ntiles;
-  uint32_t w = 4 * T + 2 + 2 * L + kMaxPairThreads / 64 + 1 + 4;
+  uint32_t w = 4 * T + 2 + 2 * L + 2 + kMaxPairThreads / 64 + 1 + 4;
   return (w + 7u) & ~7u;  // 32-byte alignment of the record array
 }
 
 template <int THREADS>
-void launch_pair_tiles_as(int form, bool digest, const PairTileArgs &a, size_t shm, hipStream_t s) {
-  const dim3 grid(a.tp.nplanes * a.tp.nscen * a.tp.nchunk), block(THREADS);
+void launch_pair_tiles_as(int form, bool digest, const PairTileArgs &a, hipStream_t s) {
+  const TileParams &tp = a.tp;
+  const size_t rec = (form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
+  const size_t shm = sizeof(uint32_t) * tp.lds_header_words + rec * tp.cap_items;
+  const dim3 grid(tp.nplanes * tp.nscen * tp.nchunk), block(THREADS);
   const bool f64 = (form == kFormFp64);
   if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false, THREADS>), grid, block, shm, s, a);
   if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true, THREADS>), grid, block, shm, s, a);
@@ -571,20 +576,18 @@ hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, cons
   a.tps = tps;
   a.scratch = scratch;
   a.arena_top = arena_top;
-  const size_t rec = (form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
-  const size_t shm = sizeof(uint32_t) * tp.lds_header_words + rec * tp.cap_items;
   switch (tp.pair_threads) {
     case 128:
-      launch_pair_tiles_as<128>(form, digest, a, shm, s);
+      launch_pair_tiles_as<128>(form, digest, a, s);
       break;
     case 256:
-      launch_pair_tiles_as<256>(form, digest, a, shm, s);
+      launch_pair_tiles_as<256>(form, digest, a, s);
       break;
     case 512:
-      launch_pair_tiles_as<512>(form, digest, a, shm, s);
+      launch_pair_tiles_as<512>(form, digest, a, s);
       break;
     case 1024:
-      launch_pair_tiles_as<1024>(form, digest, a, shm, s);
+      launch_pair_tiles_as<1024>(form, digest, a, s);
       break;
     default:
       return hipErrorInvalidValue;
