@@ -217,11 +217,12 @@ __device__ __forceinline__ bool within(const RecOf<FORM> &a, const RecOf<FORM> &
 // where the fold wraps (fz = 0 or Kz-1), a one-bucket run at the other end of the row.
 // Both runs are read unconditionally (the second one empty when there is no wrap), so
 // the lanes of a wavefront take the same path.  cur[b] is the end of bucket b in the
-// sorted array and cur[-1] = 0, so bucket b spans [cur[b-1], cur[b]).  The rows are
+// sorted array and cur[-1] the start of bucket 0 (0 for the own plane, n_own for the
+// next), so bucket b spans [cur[b-1], cur[b]).  The rows are
 // walked by one loop so that the candidate path (the exact predicate and the wavefront
 // compaction) exists once in the code.
 template <int FORM, bool DIGEST, bool CHUNKED>
-__device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FORM> *own, const RecOf<FORM> *halo,
+__device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FORM> *sorted,
                                            const uint32_t *cur_own, const uint32_t *cur_halo, uint32_t n_own,
                                            uint32_t threads, uint32_t scen, uint32_t chunk, uint32_t nchunk,
                                            ScanAcc &acc) {
@@ -230,7 +231,7 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
   const uint32_t sy = g.bz;
 #pragma unroll 1
   for (uint32_t s = threadIdx.x; s < n_own; s += threads) {
-    const RecOf<FORM> me = own[s];
+    const RecOf<FORM> me = sorted[s];
     // chunks split the plane's objects by object index: the placement order differs
     // between the chunks' own sorts, the index does not
     if (CHUNKED && me.idx % nchunk != chunk) continue;
@@ -243,26 +244,25 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
     const uint32_t zw = fz ? 0u : mz;  // the wrapped bucket when `wraps`
 #pragma unroll 1
     for (uint32_t kind = 0; kind < 6u; ++kind) {
+      const uint32_t *cur = (kind & 1u) ? cur_halo : cur_own;
+      const uint32_t row = ((fy + (kind >> 1) - 1u) & my) << sy;
+      const uint32_t b_lo = row | zlo, b_w = row | zw;
+      const uint32_t lo0 = cur[(int32_t)b_lo - 1], hi0 = cur[row | zhi];
+      const uint32_t w0 = cur[(int32_t)b_w - 1], w1 = cur[b_w];
+      const uint32_t lo1 = wraps ? w0 : 0u, hi1 = wraps ? w1 : 0u;
       const bool next_plane = (kind & 1u) != 0u;
-      const RecOf<FORM> *arr = next_plane ? halo : own;
-      const uint32_t *cur = next_plane ? cur_halo : cur_own;
       // Own-plane pairs: with one workgroup per plane each pair is taken from its lower
       // sorted position.  The in-LDS placement order within a bucket depends on atomic
       // timing, so chunks of one plane (each sorting the plane itself) cannot share that
       // rule: they take a pair from its lower object index instead.
       const bool by_index = CHUNKED && !next_plane;
       const uint32_t from = (next_plane || by_index) ? 0u : s + 1u;
-      const uint32_t row = ((fy + (kind >> 1) - 1u) & my) << sy;
-      const uint32_t b_lo = row | zlo, b_w = row | zw;
-      const uint32_t lo0 = cur[(int32_t)b_lo - 1], hi0 = cur[row | zhi];
-      const uint32_t w0 = cur[(int32_t)b_w - 1], w1 = cur[b_w];
-      const uint32_t lo1 = wraps ? w0 : 0u, hi1 = wraps ? w1 : 0u;
       uint32_t q = max(lo0, from), end = hi0;
 #pragma unroll 1
       for (int part = 0; part < 2; ++part) {
 #pragma unroll 1
         for (; q < end; ++q) {
-          const RecOf<FORM> other = arr[q];
+          const RecOf<FORM> other = sorted[q];
           if (CHUNKED && by_index && other.idx <= me.idx) continue;
           if (within<FORM>(me, other, a.sink.fp)) handle_candidate<DIGEST>(a.sink, me.idx, other.idx, acc, scen);
         }
@@ -292,8 +292,11 @@ __device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds,
   // the plane pair (own records first) lies in segment t with seg[t] <= j < seg[t+1].
   uint32_t *seg = lds;
   uint32_t *src = seg + (2u * T + 1u);     // 2T source offsets
-  uint32_t *cur_own = src + 2u * T + 1u;   // L bucket cursors, cur_own[-1] = 0
-  uint32_t *cur_next = cur_own + L + 1u;   // L bucket cursors, cur_next[-1] = 0
+  // Bucket cursors of both planes in one array [own L][1][next L] (cur_own[-1] = 0): one
+  // prefix over it gives positions in a single sorted array, own records first, and leaves
+  // cur_next[-1] = n_own, the start of the next plane's bucket 0.
+  uint32_t *cur_own = src + 2u * T + 1u;
+  uint32_t *cur_next = cur_own + L + 1u;
   uint32_t *wsum = cur_next + L;           // THREADS / 64 + 1
   uint32_t *misc = wsum + (kMaxPairThreads / 64 + 1);  // [0] scratch offset
   REC *lds_items = reinterpret_cast<REC *>(lds + tp.lds_header_words);
@@ -310,14 +313,8 @@ __device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds,
     src[t] = tile_base + o0;
     src[T + t] = tile_base + n0;
   }
-  for (uint32_t b = threadIdx.x; b < L; b += blockDim.x) {
-    cur_own[b] = 0;
-    cur_next[b] = 0;
-  }
-  if (threadIdx.x == 0) {
-    cur_own[-1] = 0;
-    cur_next[-1] = 0;
-  }
+  for (uint32_t b = threadIdx.x; b < 2u * L + 1u; b += blockDim.x) cur_own[b] = 0;
+  if (threadIdx.x == 0) cur_own[-1] = 0;
   __syncthreads();
   const uint32_t total = block_exclusive_scan(seg, 2u * T, wsum);
   const uint32_t n_own = seg[T];
@@ -335,7 +332,7 @@ __device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds,
     __syncthreads();
     if (!in_lds && !misc[1]) return;  // no room: the plane is reported, never written out of bounds
     REC *sorted = in_lds ? lds_items : static_cast<REC *>(a.scratch) + misc[0];
-    REC *own = sorted, *halo = sorted + n_own;
+    // own records at [0, n_own), the next plane's at [n_own, total)
     // Each thread reads a block of consecutive records: one binary search for the
     // block's first record, then a forward walk over the segment boundaries.
     auto find = [&](uint32_t j) -> uint32_t {
@@ -358,12 +355,14 @@ __device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds,
       // records stay in registers between the histogram and the placement pass
       REC reg[kPairRegItems];
       uint32_t lb[kPairRegItems];
-      const uint32_t jb = threadIdx.x * (uint32_t)kPairRegItems;
+      const uint32_t ipt = (total + THREADS - 1u) / THREADS;  // records per thread
+      const uint32_t jb = threadIdx.x * ipt;
+      const uint32_t je = min(total, jb + ipt);
       uint32_t t = jb < total ? find(jb) : 0u;
 #pragma unroll
       for (int k = 0; k < kPairRegItems; ++k) {
         const uint32_t j = jb + k;
-        if (j < total) {
+        if (j < je) {
           while (seg[t + 1u] <= j) ++t;
           reg[k] = items[src[t] + (j - seg[t])];
         }
@@ -371,22 +370,17 @@ __device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds,
 #pragma unroll
       for (int k = 0; k < kPairRegItems; ++k) {
         const uint32_t j = jb + k;
-        if (j < total) {
+        if (j < je) {
           lb[k] = local_bucket(reg[k]);
           atomicAdd(&(j < n_own ? cur_own : cur_next)[lb[k]], 1u);
         }
       }
       __syncthreads();
-      block_exclusive_scan(cur_own, L, wsum);
-      block_exclusive_scan(cur_next, L, wsum);
+      block_exclusive_scan(cur_own, 2u * L + 1u, wsum);
 #pragma unroll
       for (int k = 0; k < kPairRegItems; ++k) {
         const uint32_t j = jb + k;
-        if (j < total) {
-          const bool is_own = j < n_own;
-          const uint32_t pos = atomicAdd(&(is_own ? cur_own : cur_next)[lb[k]], 1u);
-          (is_own ? own : halo)[pos] = reg[k];
-        }
+        if (j < je) sorted[atomicAdd(&(j < n_own ? cur_own : cur_next)[lb[k]], 1u)] = reg[k];
       }
       __syncthreads();
     } else {
@@ -410,37 +404,29 @@ __device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds,
           for (uint32_t u = 0; u < kBatch; ++u) {
             const uint32_t j = jb + u;
             if (j >= total) continue;
-            const bool is_own = j < n_own;
-            uint32_t *cur = is_own ? cur_own : cur_next;
+            uint32_t *cur = j < n_own ? cur_own : cur_next;
             const uint32_t b = local_bucket(r[u]);
-            if (pass == 0) {
+            if (pass == 0)
               atomicAdd(&cur[b], 1u);
-            } else {
-              const uint32_t pos = atomicAdd(&cur[b], 1u);
-              (is_own ? own : halo)[pos] = r[u];
-            }
+            else
+              sorted[atomicAdd(&cur[b], 1u)] = r[u];
           }
         }
         __syncthreads();
-        if (pass == 0) {
-          block_exclusive_scan(cur_own, L, wsum);
-          block_exclusive_scan(cur_next, L, wsum);
-        }
+        if (pass == 0) block_exclusive_scan(cur_own, 2u * L + 1u, wsum);
       }
     }
     // cur[b] now holds the end of bucket b
     if (cp > 1u) {
       if (in_lds)
-        scan_plane<FORM, DIGEST, true>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, THREADS, scen,
-                                       chunk, cp, acc);
+        scan_plane<FORM, DIGEST, true>(a, lds_items, cur_own, cur_next, n_own, THREADS, scen, chunk, cp, acc);
       else
-        scan_plane<FORM, DIGEST, true>(a, own, halo, cur_own, cur_next, n_own, THREADS, scen, chunk, cp, acc);
+        scan_plane<FORM, DIGEST, true>(a, sorted, cur_own, cur_next, n_own, THREADS, scen, chunk, cp, acc);
     } else {
       if (in_lds)
-        scan_plane<FORM, DIGEST, false>(a, lds_items, lds_items + n_own, cur_own, cur_next, n_own, THREADS, scen,
-                                        0u, 1u, acc);
+        scan_plane<FORM, DIGEST, false>(a, lds_items, cur_own, cur_next, n_own, THREADS, scen, 0u, 1u, acc);
       else
-        scan_plane<FORM, DIGEST, false>(a, own, halo, cur_own, cur_next, n_own, THREADS, scen, 0u, 1u, acc);
+        scan_plane<FORM, DIGEST, false>(a, sorted, cur_own, cur_next, n_own, THREADS, scen, 0u, 1u, acc);
     }
   }
 }
