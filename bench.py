@@ -60,11 +60,11 @@ BYTES = {
 # runs and the commands are in profiles/hbm_traffic_1M_full_model.md.
 MEASURED_TRAFFIC_1M = {
     ("propagate", "fp64"): (71.46 + 58.72) * 1e6,
-    ("propagate", "split"): (67.24 + 58.72) * 1e6,
-    ("tile_partition", "fp64"): (29.44 + 48.08) * 1e6,
+    ("propagate", "split"): (67.27 + 58.72) * 1e6,
+    ("tile_partition", "fp64"): (29.44 + 48.75) * 1e6,
     ("tile_partition", "split"): (29.44 + 17.83) * 1e6,
-    ("pair_tiles", "fp64"): (100.26 + 0.06) * 1e6,
-    ("pair_tiles", "split"): (69.86 + 0.06) * 1e6,
+    ("pair_tiles", "fp64"): (100.21 + 0.06) * 1e6,
+    ("pair_tiles", "split"): (69.77 + 0.06) * 1e6,
 }
 
 
