@@ -952,28 +952,33 @@ int reserve_objects(ladds_ctx *ctx, size_t need, bool keep) {
   }
   if (keep && ctx->n) {
     const size_t n = ctx->n;
-    auto cp = [&](void *dst, const void *src, size_t elem) {
-      return hipMemcpyAsync(dst, src, elem * n, hipMemcpyDeviceToDevice, ctx->stream);
-    };
     const DeviceObjects &o = ctx->d;
-    HIPCHK(ctx, cp(fresh.rx, o.rx, 8));
-    HIPCHK(ctx, cp(fresh.ry, o.ry, 8));
-    HIPCHK(ctx, cp(fresh.rz, o.rz, 8));
-    HIPCHK(ctx, cp(fresh.vx, o.vx, 8));
-    HIPCHK(ctx, cp(fresh.vy, o.vy, 8));
-    HIPCHK(ctx, cp(fresh.vz, o.vz, 8));
-    HIPCHK(ctx, cp(fresh.radius_m, o.radius_m, 8));
-    HIPCHK(ctx, cp(fresh.aom, o.aom, 8));
-    HIPCHK(ctx, cp(fresh.bc_inv, o.bc_inv, 8));
-    HIPCHK(ctx, cp(fresh.cr, o.cr, 8));
-    HIPCHK(ctx, cp(fresh.srp64, o.srp64, 8));
-    HIPCHK(ctx, cp(fresh.drag64, o.drag64, 8));
-    HIPCHK(ctx, cp(fresh.srp32, o.srp32, 4));
-    HIPCHK(ctx, cp(fresh.drag32, o.drag32, 4));
-    HIPCHK(ctx, cp(fresh.activity, o.activity, 4));
-    HIPCHK(ctx, cp(fresh.id, o.id, 8));
-    HIPCHK(ctx, cp(fresh.parent, o.parent, 8));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    hipError_t e = hipSuccess;
+    auto cp = [&](void *dst, const void *src, size_t elem) {
+      if (e == hipSuccess) e = hipMemcpyAsync(dst, src, elem * n, hipMemcpyDeviceToDevice, ctx->stream);
+    };
+    cp(fresh.rx, o.rx, 8);
+    cp(fresh.ry, o.ry, 8);
+    cp(fresh.rz, o.rz, 8);
+    cp(fresh.vx, o.vx, 8);
+    cp(fresh.vy, o.vy, 8);
+    cp(fresh.vz, o.vz, 8);
+    cp(fresh.radius_m, o.radius_m, 8);
+    cp(fresh.aom, o.aom, 8);
+    cp(fresh.bc_inv, o.bc_inv, 8);
+    cp(fresh.cr, o.cr, 8);
+    cp(fresh.srp64, o.srp64, 8);
+    cp(fresh.drag64, o.drag64, 8);
+    cp(fresh.srp32, o.srp32, 4);
+    cp(fresh.drag32, o.drag32, 4);
+    cp(fresh.activity, o.activity, 4);
+    cp(fresh.id, o.id, 8);
+    cp(fresh.parent, o.parent, 8);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      free_objects(fresh);
+      return ctx->hipfail(e, "object copy");
+    }
   }
   fresh.n = ctx->d.n;
   free_objects(ctx->d);
