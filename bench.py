@@ -58,6 +58,16 @@ BYTES = {
 # HBM bytes per launch at 1,048,576 objects (read + write), measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by a 256 MiB copy kernel, default plane-tile shape; the
 # runs and the commands are in profiles/hbm_traffic_1M_full_model.md.
+# FP64 and FP32 VALU flops per object-step of the propagation kernel (full force model),
+# from rocprofv3 SQ_INSTS_VALU_FLOPS_{FP64,FP32} (+ the _TRANS counters), per wavefront
+# instruction and lane, on the 1M-object workload.
+MEASURED_FLOPS_PER_OBJECT = {
+    ("propagate", "fp64"): (794.0 + 15.0, 0.0),
+    ("propagate", "split"): (551.0 + 6.0, 249.0 + 12.0),
+}
+FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32 (non-packed issue is half of this)
+
 MEASURED_TRAFFIC_1M = {
     ("propagate", "fp64"): (71.46 + 58.72) * 1e6,
     ("propagate", "split"): (67.27 + 58.72) * 1e6,
@@ -151,17 +161,33 @@ def cpu_baseline(pop, steps):
     import oracle_lib as ol
 
     threads = min(16, len(os.sched_getaffinity(0)))
-    n = pop.n
-    r, v = pop.r(), pop.v()
     cfg = ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0)
-    t0 = time.perf_counter()
-    for s in range(steps):
-        r, v = ol.propagate(r, v, s * 10.0, 10.0, 0xFF, 1, pop.aom, pop.bc_inv, pop.cr, pop.activity, threads)
-        ol.pairs(ol.ObjectSet(r, v, pop.radius_m, pop.activity, pop.id, pop.parent), cfg, "grid", threads)
-    el = time.perf_counter() - t0
+
+    def run(n, nsteps, nthreads):
+        sel = slice(0, n)
+        r, v = pop.r()[sel], pop.v()[sel]
+        t0 = time.perf_counter()
+        for s in range(nsteps):
+            r, v = ol.propagate(r, v, s * 10.0, 10.0, 0xFF, 1, pop.aom[sel], pop.bc_inv[sel], pop.cr[sel],
+                                pop.activity[sel], nthreads)
+            ol.pairs(ol.ObjectSet(r, v, pop.radius_m[sel], pop.activity[sel], pop.id[sel], pop.parent[sel]), cfg,
+                     "grid", nthreads)
+        return time.perf_counter() - t0
+
+    n = pop.n
+    el = run(n, steps, threads)
+    n1, steps1 = min(n, 1 << 17), 5
+    el1 = run(n1, steps1, 1)
+    model = "unknown"
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": n * steps / el, "unit": "object-steps/s", "cores": threads, "kind": "port",
             "sample": f"C oracle (OpenMP, -O2, no FMA contraction): {n} objects x {steps} steps of the same "
-                      f"workload (propagate + linked-cell scan), {el:.1f} s on {threads} host threads"}
+                      f"workload (propagate + linked-cell scan), {el:.1f} s on {threads} host threads",
+            "cpu_model": model, "nproc": os.cpu_count(), "threads_available": len(os.sched_getaffinity(0)),
+            "single_thread": {"value": n1 * steps1 / el1, "sample": f"{n1} objects x {steps1} steps, {el1:.1f} s"}}
 
 
 def adjoint_leg(la, reps=3):
@@ -385,9 +411,18 @@ def main():
             return None
         achieved = per_obj * a.objects / (k["avg_us"] * 1e-6) / 1e9
         traffic = MEASURED_TRAFFIC_1M.get((name, form)) if a.objects == 1 << 20 else None
-        return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_us": k["avg_us"],
-                "bytes_per_object": per_obj}
+        out = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_us": k["avg_us"],
+               "bytes_per_object": per_obj}
+        flops = MEASURED_FLOPS_PER_OBJECT.get((name, form))
+        if flops:
+            f64 = flops[0] * a.objects / (k["avg_us"] * 1e-6) / 1e12
+            f32 = flops[1] * a.objects / (k["avg_us"] * 1e-6) / 1e12
+            out["valu"] = {"fp64_tflops": f64, "fp32_tflops": f32, "peak_fp64": FP64_PEAK_TFLOPS,
+                           "peak_fp32": FP32_PEAK_TFLOPS,
+                           "frac": f64 / FP64_PEAK_TFLOPS + f32 / FP32_PEAK_TFLOPS,
+                           "flops_per_object": list(flops)}
+        return out
 
     kern = m["kernels"]
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
