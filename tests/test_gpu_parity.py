@@ -430,3 +430,30 @@ def test_chunked_planes_keep_the_pair_set(chunks, precision):
     assert oh.shape[0] > 1500 * 1499 // 2
     _pair_equal(ol.sort_pairs(hits), oh)
     _pair_equal(ol.sort_pairs(ev), oe)
+
+
+@pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
+def test_lds_and_arena_planes_in_one_scan(precision):
+    """A sparse LEO population plus a 12,000-object cluster: the cluster's plane pair exceeds
+    the LDS of the pair kernel (sorted in the global arena) while every other plane is
+    sorted in LDS, in the same launch; the pair set equals the oracle's."""
+    pop = la.generate_population(0, 0x1ADD5 + 31, 20000, planted_pairs=40)
+    rng = np.random.default_rng(9)
+    m = 12000
+    c = np.array([7000.0, 50.0, 20.0]) + rng.uniform(-1.5, 1.5, (m, 3))
+    cv = np.array([0.0, 7.5, 0.0]) + rng.normal(0, 0.01, (m, 3))
+    n = pop.n + m
+    r = np.concatenate([pop.r(), c])
+    v = np.concatenate([pop.v(), cv])
+    radius = np.concatenate([pop.radius_m, np.full(m, 0.05)])
+    with engine(dt=10.0, components=la.KEP, precision=precision, cutoff_km=10.0,
+                threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=0.05) as eng:
+        eng.set_objects(rx=r[:, 0], ry=r[:, 1], rz=r[:, 2], vx=v[:, 0], vy=v[:, 1], vz=v[:, 2], radius_m=radius)
+        hits, ev = eng.conjunctions()
+        cnt, dig, _, _ = eng.conjunction_digest()
+    objs = ol.ObjectSet(r, v, radius)
+    oh, oe = ol.pairs(objs, ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=0.05), method="grid")
+    assert oh.shape[0] > 1000 and n == 32000
+    _pair_equal(ol.sort_pairs(hits), oh)
+    _pair_equal(ol.sort_pairs(ev), oe)
+    assert cnt == oh.shape[0] and dig == ol.digest(oh)
