@@ -235,105 +235,187 @@ __device__ __forceinline__ void store_state(const State &s, double *base, size_t
   base[5 * n + i] = s.vz;
 }
 
-__global__ __launch_bounds__(64) void k_adjoint_forward(AdjointArgs a) {
-  __shared__ AtmTable atm;
-  stage_atm(atm);
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  const size_t n = a.n;
-  State s{a.r0[i], a.r0[n + i], a.r0[2 * n + i], a.v0[i], a.v0[n + i], a.v0[2 * n + i]};
-  const ObjParams p = object_params(a, i);
-  const int nsteps = p.activity < 0 ? 0 : a.nsteps;  // dummies keep their state
-  for (int step = 0; step < a.nsteps; ++step) {
-    if (step % a.every == 0) store_state(s, a.ck + (size_t)(step / a.every) * 6 * n, n, i);
-    if (step < nsteps) yoshida_step(s, p, a.mask, a.cdt, a.ddt, a.env + 3 * step, atm);
-  }
-  a.rT[i] = s.x;
-  a.rT[n + i] = s.y;
-  a.rT[2 * n + i] = s.z;
-  a.vT[i] = s.vx;
-  a.vT[n + i] = s.vy;
-  a.vT[2 * n + i] = s.vz;
+// Four wavefronts per workgroup share 64 objects: wavefront w evaluates the component
+// group kGroupMask[w] of every acceleration (and of its adjoint) for the 64 objects, the
+// four partial results are summed through LDS in a fixed order, and every wavefront then
+// carries the same integrator state.  One object's force evaluation is thus spread over
+// the four SIMDs of a CU (the sweep over steps is sequential, so this is where the
+// parallelism is).  Groups are balanced by cost.
+constexpr int kAdjWaves = 4;
+constexpr int kAdjLanes = 64;
+__device__ constexpr unsigned kGroupMask[kAdjWaves] = {kKepBit | kJ2Bit | kSrpBit, kDragBit, kC22Bit | kS22Bit,
+                                                       kSolBit | kLunBit};
+
+// Partial sums of the four wavefronts, double-buffered so that one barrier per exchange
+// suffices: a buffer is rewritten only after the next exchange's barrier, which every
+// wavefront reaches after reading it.
+struct GroupExchange {
+  double buf[2][kAdjWaves][8][kAdjLanes];
+  int parity;
+};
+
+template <int NV>
+__device__ __forceinline__ void group_sum(GroupExchange &x, int &parity, double (&v)[NV]) {
+  const int w = threadIdx.x / kAdjLanes, l = threadIdx.x % kAdjLanes;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) x.buf[parity][w][k][l] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    v[k] = ((x.buf[parity][0][k][l] + x.buf[parity][1][k][l]) + x.buf[parity][2][k][l]) + x.buf[parity][3][k][l];
+  parity ^= 1;
 }
 
-__global__ __launch_bounds__(64) void k_adjoint_backward(AdjointArgs a) {
+__device__ __forceinline__ void group_acceleration(const State &s, const ObjParams &p, unsigned gmask,
+                                                   const SubstepEnv<double> &env, const AtmTable &atm,
+                                                   GroupExchange &x, int &parity, double &ax, double &ay,
+                                                   double &az) {
+  const SubstepEnv<float> unused{};
+  double v[3];
+  acceleration<kFormFp64>(s.x, s.y, s.z, s.vx, s.vy, s.vz, p, gmask, env, unused, atm, v[0], v[1], v[2]);
+  group_sum<3>(x, parity, v);
+  ax = v[0];
+  ay = v[1];
+  az = v[2];
+}
+
+// One Yoshida step (the composition of propagate.hip) with group-split accelerations;
+// `moving` false keeps the state (dummies) while the wavefront still takes part in the
+// exchanges.
+__device__ __forceinline__ void group_step(State &s, const ObjParams &p, unsigned gmask, bool moving,
+                                           const double *cdt, const double *ddt, const SubstepEnv<double> *env,
+                                           const AtmTable &atm, GroupExchange &x, int &parity) {
+  const double go = moving ? 1.0 : 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s.x += go * cdt[k] * s.vx;
+    s.y += go * cdt[k] * s.vy;
+    s.z += go * cdt[k] * s.vz;
+    double ax, ay, az;
+    group_acceleration(s, p, gmask, env[k], atm, x, parity, ax, ay, az);
+    s.vx += go * ddt[k] * ax;
+    s.vy += go * ddt[k] * ay;
+    s.vz += go * ddt[k] * az;
+  }
+  s.x += go * cdt[3] * s.vx;
+  s.y += go * cdt[3] * s.vy;
+  s.z += go * cdt[3] * s.vz;
+}
+
+__global__ __launch_bounds__(kAdjWaves * kAdjLanes) void k_adjoint_forward(AdjointArgs a) {
   __shared__ AtmTable atm;
+  __shared__ GroupExchange x;
   stage_atm(atm);
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  const size_t n = a.n;
-  double lx = a.g_rT ? a.g_rT[i] : 0.0, ly = a.g_rT ? a.g_rT[n + i] : 0.0, lz = a.g_rT ? a.g_rT[2 * n + i] : 0.0;
-  double lvx = a.g_vT ? a.g_vT[i] : 0.0, lvy = a.g_vT ? a.g_vT[n + i] : 0.0,
-         lvz = a.g_vT ? a.g_vT[2 * n + i] : 0.0;
+  const int w = threadIdx.x / kAdjLanes, l = threadIdx.x % kAdjLanes;
+  const size_t n = a.n, i = (size_t)blockIdx.x * kAdjLanes + l;
+  const bool live = i < n;
+  const size_t ii = live ? i : 0;  // lanes past the end compute on object 0 and write nothing
+  State s{a.r0[ii], a.r0[n + ii], a.r0[2 * n + ii], a.v0[ii], a.v0[n + ii], a.v0[2 * n + ii]};
+  const ObjParams p = object_params(a, ii);
+  const bool moving = p.activity >= 0;  // dummies keep their state
+  const unsigned gmask = a.mask & kGroupMask[w];
+  int parity = 0;
+  for (int step = 0; step < a.nsteps; ++step) {
+    if (w == 0 && live && step % a.every == 0) store_state(s, a.ck + (size_t)(step / a.every) * 6 * n, n, i);
+    group_step(s, p, gmask, moving, a.cdt, a.ddt, a.env + 3 * step, atm, x, parity);
+  }
+  if (w == 0 && live) {
+    a.rT[i] = s.x;
+    a.rT[n + i] = s.y;
+    a.rT[2 * n + i] = s.z;
+    a.vT[i] = s.vx;
+    a.vT[n + i] = s.vy;
+    a.vT[2 * n + i] = s.vz;
+  }
+}
+
+__global__ __launch_bounds__(kAdjWaves * kAdjLanes) void k_adjoint_backward(AdjointArgs a) {
+  __shared__ AtmTable atm;
+  __shared__ GroupExchange x;
+  stage_atm(atm);
+  const int w = threadIdx.x / kAdjLanes, l = threadIdx.x % kAdjLanes;
+  const size_t n = a.n, i = (size_t)blockIdx.x * kAdjLanes + l;
+  const bool live = i < n;
+  const size_t ii = live ? i : 0;
+  double lx = a.g_rT ? a.g_rT[ii] : 0.0, ly = a.g_rT ? a.g_rT[n + ii] : 0.0, lz = a.g_rT ? a.g_rT[2 * n + ii] : 0.0;
+  double lvx = a.g_vT ? a.g_vT[ii] : 0.0, lvy = a.g_vT ? a.g_vT[n + ii] : 0.0,
+         lvz = a.g_vT ? a.g_vT[2 * n + ii] : 0.0;
   double lbc = 0.0;
-  const ObjParams p = object_params(a, i);
-  if (p.activity >= 0) {
-    for (int seg = a.nck - 1; seg >= 0; --seg) {
-      const int s0 = seg * a.every;
-      const int s1 = min(s0 + a.every, a.nsteps);
-      // recompute the step-start states of the segment
-      State s;
-      load_state(s, a.ck + (size_t)seg * 6 * n, n, i);
-      for (int step = s0; step < s1; ++step) {
-        store_state(s, a.work + (size_t)(step - s0) * 6 * n, n, i);
-        if (step + 1 < s1) yoshida_step(s, p, a.mask, a.cdt, a.ddt, a.env + 3 * step, atm);
+  const ObjParams p = object_params(a, ii);
+  const bool moving = p.activity >= 0;
+  const unsigned gmask = a.mask & kGroupMask[w];
+  int parity = 0;
+  for (int seg = a.nck - 1; seg >= 0; --seg) {
+    const int s0 = seg * a.every;
+    const int s1 = min(s0 + a.every, a.nsteps);
+    // Recompute the step-start states of the segment.  Every wavefront holds the same
+    // values (same operations on the same inputs) and stores them to the same addresses,
+    // so each thread reads back exactly what it wrote.
+    State s;
+    load_state(s, a.ck + (size_t)seg * 6 * n, n, ii);
+    for (int step = s0; step < s1; ++step) {
+      if (live) store_state(s, a.work + (size_t)(step - s0) * 6 * n, n, i);
+      if (step + 1 < s1) group_step(s, p, gmask, moving, a.cdt, a.ddt, a.env + 3 * step, atm, x, parity);
+    }
+    for (int step = s1 - 1; step >= s0; --step) {
+      if (live) load_state(s, a.work + (size_t)(step - s0) * 6 * n, n, i);
+      const SubstepEnv<double> *env = a.env + 3 * step;
+      // forward through the step, keeping the kick inputs (r_k, v_k)
+      double rx[3], ry[3], rz[3], ux[3], uy[3], uz[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        s.x += a.cdt[k] * s.vx;
+        s.y += a.cdt[k] * s.vy;
+        s.z += a.cdt[k] * s.vz;
+        rx[k] = s.x;
+        ry[k] = s.y;
+        rz[k] = s.z;
+        ux[k] = s.vx;
+        uy[k] = s.vy;
+        uz[k] = s.vz;
+        if (k < 2) {
+          double ax, ay, az;
+          group_acceleration(s, p, gmask, env[k], atm, x, parity, ax, ay, az);
+          s.vx += a.ddt[k] * ax;
+          s.vy += a.ddt[k] * ay;
+          s.vz += a.ddt[k] * az;
+        }
       }
-      for (int step = s1 - 1; step >= s0; --step) {
-        load_state(s, a.work + (size_t)(step - s0) * 6 * n, n, i);
-        const SubstepEnv<double> *env = a.env + 3 * step;
-        // forward through the step, keeping the kick inputs (r_k, v_k)
-        double rx[3], ry[3], rz[3], ux[3], uy[3], uz[3];
-        const SubstepEnv<float> unused{};
+      // reverse sweep
+      lvx += a.cdt[3] * lx;
+      lvy += a.cdt[3] * ly;
+      lvz += a.cdt[3] * lz;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          s.x += a.cdt[k] * s.vx;
-          s.y += a.cdt[k] * s.vy;
-          s.z += a.cdt[k] * s.vz;
-          rx[k] = s.x;
-          ry[k] = s.y;
-          rz[k] = s.z;
-          ux[k] = s.vx;
-          uy[k] = s.vy;
-          uz[k] = s.vz;
-          if (k < 2) {
-            double ax, ay, az;
-            acceleration<kFormFp64>(s.x, s.y, s.z, s.vx, s.vy, s.vz, p, a.mask, env[k], unused, atm, ax, ay, az);
-            s.vx += a.ddt[k] * ax;
-            s.vy += a.ddt[k] * ay;
-            s.vz += a.ddt[k] * az;
-          }
-        }
-        // reverse sweep
-        lvx += a.cdt[3] * lx;
-        lvy += a.cdt[3] * ly;
-        lvz += a.cdt[3] * lz;
-#pragma unroll
-        for (int k = 2; k >= 0; --k) {
-          const double wx = a.ddt[k] * lvx, wy = a.ddt[k] * lvy, wz = a.ddt[k] * lvz;
-          double gx, gy, gz, gvx, gvy, gvz, gbc;
-          acceleration_vjp(rx[k], ry[k], rz[k], ux[k], uy[k], uz[k], p, a.mask, env[k], atm, wx, wy, wz, gx, gy,
-                           gz, gvx, gvy, gvz, gbc);
-          lx += gx;
-          ly += gy;
-          lz += gz;
-          lvx += gvx;
-          lvy += gvy;
-          lvz += gvz;
-          lbc += gbc;
-          lvx += a.cdt[k] * lx;
-          lvy += a.cdt[k] * ly;
-          lvz += a.cdt[k] * lz;
-        }
+      for (int k = 2; k >= 0; --k) {
+        const double wx = a.ddt[k] * lvx, wy = a.ddt[k] * lvy, wz = a.ddt[k] * lvz;
+        double g[7];
+        acceleration_vjp(rx[k], ry[k], rz[k], ux[k], uy[k], uz[k], p, gmask, env[k], atm, wx, wy, wz, g[0], g[1],
+                         g[2], g[3], g[4], g[5], g[6]);
+        group_sum<7>(x, parity, g);
+        lx += g[0];
+        ly += g[1];
+        lz += g[2];
+        lvx += g[3];
+        lvy += g[4];
+        lvz += g[5];
+        lbc += g[6];
+        lvx += a.cdt[k] * lx;
+        lvy += a.cdt[k] * ly;
+        lvz += a.cdt[k] * lz;
       }
     }
   }
-  a.g_r0[i] = lx;
-  a.g_r0[n + i] = ly;
-  a.g_r0[2 * n + i] = lz;
-  a.g_v0[i] = lvx;
-  a.g_v0[n + i] = lvy;
-  a.g_v0[2 * n + i] = lvz;
-  if (a.g_bc) a.g_bc[i] = lbc;
+  if (w == 0 && live) {
+    // dummies: no gradient flows through a deleted object
+    const double keep = moving ? 1.0 : 0.0;
+    a.g_r0[i] = moving ? lx : (a.g_rT ? a.g_rT[i] : 0.0);
+    a.g_r0[n + i] = moving ? ly : (a.g_rT ? a.g_rT[n + i] : 0.0);
+    a.g_r0[2 * n + i] = moving ? lz : (a.g_rT ? a.g_rT[2 * n + i] : 0.0);
+    a.g_v0[i] = moving ? lvx : (a.g_vT ? a.g_vT[i] : 0.0);
+    a.g_v0[n + i] = moving ? lvy : (a.g_vT ? a.g_vT[n + i] : 0.0);
+    a.g_v0[2 * n + i] = moving ? lvz : (a.g_vT ? a.g_vT[2 * n + i] : 0.0);
+    if (a.g_bc) a.g_bc[i] = keep * lbc;
+  }
 }
 
 }  // namespace ladds
@@ -451,8 +533,8 @@ int ladds_adjoint_forward(ladds_adjoint *p, const double *r0, const double *v0, 
   a.rT = rT;
   a.vT = vT;
   a.ck = checkpoints;
-  const unsigned blocks = (unsigned)((a.n + 63) / 64);
-  hipLaunchKernelGGL(k_adjoint_forward, dim3(blocks), dim3(64), 0, (hipStream_t)stream, a);
+  const unsigned blocks = (unsigned)((a.n + kAdjLanes - 1) / kAdjLanes);
+  hipLaunchKernelGGL(k_adjoint_forward, dim3(blocks), dim3(kAdjWaves * kAdjLanes), 0, (hipStream_t)stream, a);
   ADJCHK(p, hipGetLastError());
   return LADDS_OK;
 }
@@ -474,8 +556,8 @@ int ladds_adjoint_backward(ladds_adjoint *p, const double *aom, const double *bc
   a.g_r0 = g_r0;
   a.g_v0 = g_v0;
   a.g_bc = g_bc;
-  const unsigned blocks = (unsigned)((a.n + 63) / 64);
-  hipLaunchKernelGGL(k_adjoint_backward, dim3(blocks), dim3(64), 0, (hipStream_t)stream, a);
+  const unsigned blocks = (unsigned)((a.n + kAdjLanes - 1) / kAdjLanes);
+  hipLaunchKernelGGL(k_adjoint_backward, dim3(blocks), dim3(kAdjWaves * kAdjLanes), 0, (hipStream_t)stream, a);
   ADJCHK(p, hipGetLastError());
   return LADDS_OK;
 }
