@@ -431,12 +431,25 @@ __device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds,
   }
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD b mod 8), each
+// XCD with its own L2.  Plane pair w reads plane w+1 again as its halo, so runs of
+// kXcdRun consecutive plane pairs are given to one XCD (the runs still alternate over the
+// XCDs, which keeps their loads balanced), and most halo reads find the records the
+// neighbouring plane pair loaded in that L2.  Applies when G is a multiple of 8 kXcdRun.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdRun = 8;
+__device__ __forceinline__ uint32_t xcd_local_item(uint32_t b, uint32_t g) {
+  if (g % (kXcds * kXcdRun)) return b;
+  const uint32_t xcd = b % kXcds, slot = b / kXcds;
+  return (slot / kXcdRun) * (kXcds * kXcdRun) + xcd * kXcdRun + slot % kXcdRun;
+}
+
 // One workgroup per plane pair (grid nplanes * nscen * nchunk).
 template <int FORM, bool DIGEST, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
   extern __shared__ uint32_t lds[];
   ScanAcc acc;
-  pair_plane<FORM, DIGEST, THREADS>(a, lds, blockIdx.x, acc);
+  pair_plane<FORM, DIGEST, THREADS>(a, lds, xcd_local_item(blockIdx.x, gridDim.x), acc);
   flush_counters<DIGEST>(a.sink, acc);
 }
 
