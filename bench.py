@@ -71,10 +71,10 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32 (non-packed issue is half of this
 MEASURED_TRAFFIC_1M = {
     ("propagate", "fp64"): (71.46 + 58.72) * 1e6,
     ("propagate", "split"): (67.27 + 58.72) * 1e6,
-    ("tile_partition", "fp64"): (29.44 + 48.75) * 1e6,
+    ("tile_partition", "fp64"): (29.44 + 48.08) * 1e6,
     ("tile_partition", "split"): (29.44 + 17.83) * 1e6,
-    ("pair_tiles", "fp64"): (100.21 + 0.06) * 1e6,
-    ("pair_tiles", "split"): (69.77 + 0.06) * 1e6,
+    ("pair_tiles", "fp64"): (50.10 + 0.06) * 1e6,
+    ("pair_tiles", "split"): (31.74 + 0.06) * 1e6,
 }
 
 
