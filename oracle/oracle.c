@@ -53,8 +53,24 @@ void or_sun_position(double t, double out[3]) {
   out[2] = r * sin(L) * se;
 }
 
-void or_moon_position(double t, double out[3]) {
-  /* Montenbruck & Gill (2000) eq. 3.48-3.50: low-precision lunar series. */
+/* Amplitudes of the low-precision lunar series, Montenbruck & Gill (2000) eq. 3.48-3.50,
+ * in the order the terms are summed in or_moon_position_series: longitude and latitude
+ * in arcsec, distance in km (the mean distance first). */
+static const double MOON_LAM[OR_MOON_NLAM] = {22640.0, 769.0, -4586.0, 2370.0, -668.0, -412.0, -212.0,
+                                              -206.0,  192.0, -165.0,  148.0,  -125.0, -110.0, -55.0};
+static const double MOON_BETA[OR_MOON_NBETA] = {18520.0, -526.0, 44.0, -31.0, -25.0, -23.0, 21.0, 11.0};
+static const double MOON_R[OR_MOON_NR] = {385000.0, -20905.0, -3699.0, -2956.0, -570.0,
+                                          246.0,    -205.0,   -171.0,  -152.0};
+
+void or_moon_series_default(double lam[OR_MOON_NLAM], double beta[OR_MOON_NBETA], double r[OR_MOON_NR]) {
+  memcpy(lam, MOON_LAM, sizeof(MOON_LAM));
+  memcpy(beta, MOON_BETA, sizeof(MOON_BETA));
+  memcpy(r, MOON_R, sizeof(MOON_R));
+}
+
+void or_moon_position_series(double t, const double *A, const double *B, const double *R, double out[3]) {
+  /* Montenbruck & Gill (2000) eq. 3.48-3.50: fundamental arguments (mean longitude referred
+   * to the J2000 equinox by the -1.3972 T precession term), then the three series. */
   const double T = t / SEC_PER_CENTURY;
   const double L0 = (218.31617 + 481267.88088 * T - 1.3972 * T) * DEG;
   const double l = (134.96292 + 477198.86753 * T) * DEG;
@@ -62,19 +78,19 @@ void or_moon_position(double t, double out[3]) {
   const double F = (93.27283 + 483202.01873 * T) * DEG;
   const double D = (297.85027 + 445267.11135 * T) * DEG;
   const double lam =
-      L0 + (22640.0 * sin(l) + 769.0 * sin(2.0 * l) - 4586.0 * sin(l - 2.0 * D) + 2370.0 * sin(2.0 * D) -
-            668.0 * sin(lp) - 412.0 * sin(2.0 * F) - 212.0 * sin(2.0 * l - 2.0 * D) - 206.0 * sin(l + lp - 2.0 * D) +
-            192.0 * sin(l + 2.0 * D) - 165.0 * sin(lp - 2.0 * D) + 148.0 * sin(l - lp) - 125.0 * sin(D) -
-            110.0 * sin(l + lp) - 55.0 * sin(2.0 * F - 2.0 * D)) *
+      L0 + (A[0] * sin(l) + A[1] * sin(2.0 * l) + A[2] * sin(l - 2.0 * D) + A[3] * sin(2.0 * D) + A[4] * sin(lp) +
+            A[5] * sin(2.0 * F) + A[6] * sin(2.0 * l - 2.0 * D) + A[7] * sin(l + lp - 2.0 * D) +
+            A[8] * sin(l + 2.0 * D) + A[9] * sin(lp - 2.0 * D) + A[10] * sin(l - lp) + A[11] * sin(D) +
+            A[12] * sin(l + lp) + A[13] * sin(2.0 * F - 2.0 * D)) *
                ARCSEC;
   const double beta =
-      (18520.0 * sin(F + lam - L0 + (412.0 * sin(2.0 * F) + 541.0 * sin(lp)) * ARCSEC) - 526.0 * sin(F - 2.0 * D) +
-       44.0 * sin(l + F - 2.0 * D) - 31.0 * sin(-l + F - 2.0 * D) - 25.0 * sin(-2.0 * l + F) -
-       23.0 * sin(lp + F - 2.0 * D) + 21.0 * sin(-l + F) + 11.0 * sin(-lp + F - 2.0 * D)) *
+      (B[0] * sin(F + lam - L0 + (412.0 * sin(2.0 * F) + 541.0 * sin(lp)) * ARCSEC) + B[1] * sin(F - 2.0 * D) +
+       B[2] * sin(l + F - 2.0 * D) + B[3] * sin(-l + F - 2.0 * D) + B[4] * sin(-2.0 * l + F) +
+       B[5] * sin(lp + F - 2.0 * D) + B[6] * sin(-l + F) + B[7] * sin(-lp + F - 2.0 * D)) *
       ARCSEC;
-  const double r = 385000.0 - 20905.0 * cos(l) - 3699.0 * cos(2.0 * D - l) - 2956.0 * cos(2.0 * D) -
-                   570.0 * cos(2.0 * l) + 246.0 * cos(2.0 * l - 2.0 * D) - 205.0 * cos(lp - 2.0 * D) -
-                   171.0 * cos(l + 2.0 * D) - 152.0 * cos(l + lp - 2.0 * D);
+  const double r = R[0] + R[1] * cos(l) + R[2] * cos(2.0 * D - l) + R[3] * cos(2.0 * D) + R[4] * cos(2.0 * l) +
+                   R[5] * cos(2.0 * l - 2.0 * D) + R[6] * cos(lp - 2.0 * D) + R[7] * cos(l + 2.0 * D) +
+                   R[8] * cos(l + lp - 2.0 * D);
   const double xe = r * cos(lam) * cos(beta);
   const double ye = r * sin(lam) * cos(beta);
   const double ze = r * sin(beta);
@@ -83,6 +99,8 @@ void or_moon_position(double t, double out[3]) {
   out[1] = ye * ce - ze * se;
   out[2] = ye * se + ze * ce;
 }
+
+void or_moon_position(double t, double out[3]) { or_moon_position_series(t, MOON_LAM, MOON_BETA, MOON_R, out); }
 
 double or_earth_rotation_angle(double t) {
   double th = fmod(THETA_J2000 + OMEGA_EARTH * t, TWO_PI);

@@ -60,9 +60,20 @@ double or_constant(const char *name);
 
 /* ---- ephemerides and environment (uniform per sub-step) ----
  * t: seconds since J2000.0 (2000-01-01 12:00 TT), TT ~ UT1 assumed.
- * parity unpinned against OrbitPropagator; pinned by physical invariants in tests. */
+ * parity unpinned against OrbitPropagator (its ephemeris is not in the tree); the Sun is
+ * pinned by the J2000 almanac position, the Moon by a published worked example (Meeus
+ * Example 47.a) and by physical invariants in the tests. */
 void or_sun_position(double t, double out[3]);  /* km, EME2000 (M&G eq. 3.46-3.47) */
 void or_moon_position(double t, double out[3]); /* km, EME2000 (M&G eq. 3.48-3.50) */
+/* The lunar series with caller-supplied amplitudes (longitude and latitude in arcsec,
+ * distance in km; or_moon_series_default gives the published ones).  Lets the tests plant
+ * a mistyped term and show that the pins catch it.  or_moon_position is pinned by the
+ * worked example of Meeus, Astronomical Algorithms (2nd ed. 1998) Example 47.a. */
+#define OR_MOON_NLAM 14
+#define OR_MOON_NBETA 8
+#define OR_MOON_NR 9
+void or_moon_series_default(double lam[OR_MOON_NLAM], double beta[OR_MOON_NBETA], double r[OR_MOON_NR]);
+void or_moon_position_series(double t, const double *lam, const double *beta, const double *r, double out[3]);
 double or_earth_rotation_angle(double t);       /* rad, GMST(J2000) + omega_E t, reduced to [0, 2pi) */
 double or_density(double h_km);                 /* kg/m^3, Vallado Table 8-4 exponential model */
 

@@ -46,6 +46,8 @@ def _load():
     lib.or_constant.argtypes = [C.c_char_p]
     for f in ("or_sun_position", "or_moon_position"):
         getattr(lib, f).argtypes = [C.c_double, _dp]
+    lib.or_moon_series_default.argtypes = [_dp, _dp, _dp]
+    lib.or_moon_position_series.argtypes = [C.c_double, _dp, _dp, _dp, _dp]
     lib.or_earth_rotation_angle.restype = C.c_double
     lib.or_earth_rotation_angle.argtypes = [C.c_double]
     lib.or_density.restype = C.c_double
@@ -90,6 +92,21 @@ def sun(t: float) -> np.ndarray:
 def moon(t: float) -> np.ndarray:
     out = np.zeros(3)
     lib.or_moon_position(t, _p(out))
+    return out
+
+
+def moon_series_default():
+    """Published amplitudes of the lunar series: (longitude [arcsec], latitude [arcsec], distance [km])."""
+    lam, beta, r = np.zeros(14), np.zeros(8), np.zeros(9)
+    lib.or_moon_series_default(_p(lam), _p(beta), _p(r))
+    return lam, beta, r
+
+
+def moon_with(t: float, lam, beta, r) -> np.ndarray:
+    """The oracle's lunar series evaluated with the given amplitudes (planted-mistake tests)."""
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (lam, beta, r)]
+    out = np.zeros(3)
+    lib.or_moon_position_series(t, *[_p(a) for a in arrs], _p(out))
     return out
 
 

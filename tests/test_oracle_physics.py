@@ -281,6 +281,91 @@ def test_sun_and_moon_invariants():
     assert ang < 8.0
 
 
+# Meeus, Astronomical Algorithms (2nd ed. 1998), Example 47.a: the Moon on 1992 April 12,
+# 0h TD (JDE 2448724.5), geometric ecliptic coordinates referred to the mean equinox of date.
+MEEUS_47A_JDE = 2448724.5
+MEEUS_47A = {"lambda_deg": 133.162655, "beta_deg": -3.229126, "distance_km": 368409.7}
+
+
+def _ecliptic(m):
+    """EME2000 vector -> (longitude, latitude) [deg] on the J2000 ecliptic, distance [km]."""
+    eps = ol.constant("obliquity")
+    x, y, z = m
+    ye = y * math.cos(eps) + z * math.sin(eps)
+    ze = -y * math.sin(eps) + z * math.cos(eps)
+    r = math.sqrt(x * x + y * y + z * z)
+    return math.degrees(math.atan2(ye, x)) % 360.0, math.degrees(math.asin(ze / r)), r
+
+
+def _meeus_47a_errors(moon_at):
+    """Differences of a lunar position function from Meeus Example 47.a.  The series is referred
+    to the J2000 equinox (its -1.3972 T term), Meeus to the equinox of date: the longitude is
+    carried over by the general precession in longitude p_A = 5029.0966" T + 1.11113" T^2
+    (IAU 1976); the ecliptic's own motion (~0.001 deg here) is below the tolerances."""
+    t = (MEEUS_47A_JDE - 2451545.0) * 86400.0
+    T = t / 3155760000.0
+    p_a = (5029.0966 * T + 1.11113 * T * T) / 3600.0
+    lam, beta, r = _ecliptic(moon_at(t))
+    return (lam - (MEEUS_47A["lambda_deg"] - p_a), beta - MEEUS_47A["beta_deg"], r - MEEUS_47A["distance_km"])
+
+
+# Tolerances of the low-precision series against the full theory (M&G Sec. 3.3.2 quote a few
+# arcminutes and a few hundred km); the oracle lands at 0.005 deg, 0.002 deg and 51 km.
+MOON_TOL = {"lambda_deg": 0.1, "beta_deg": 0.03, "distance_km": 150.0}
+
+
+def _meeus_ok(moon_at):
+    dl, db, dr = _meeus_47a_errors(moon_at)
+    return abs(dl) < MOON_TOL["lambda_deg"] and abs(db) < MOON_TOL["beta_deg"] and abs(dr) < MOON_TOL["distance_km"]
+
+
+def _areal_velocity_spread(moon_at, t0=0.0):
+    """r^2 dlambda/dt (Kepler's second law) over one sidereal month, relative to its median.
+    Solar perturbations make the real Moon's vary by about +-1%."""
+    h = []
+    for k in range(200):
+        t = t0 + k * 27.321661 * 86400.0 / 200
+        l0, _, r0 = _ecliptic(moon_at(t - 1800.0))
+        l1, _, r1 = _ecliptic(moon_at(t + 1800.0))
+        dl = (l1 - l0 + 180.0) % 360.0 - 180.0
+        h.append((0.5 * (r0 + r1)) ** 2 * dl)
+    h = np.array(h) / np.median(h)
+    return h.min(), h.max()
+
+
+def test_moon_meeus_example_47a():
+    dl, db, dr = _meeus_47a_errors(ol.moon)
+    assert abs(dl) < MOON_TOL["lambda_deg"], dl
+    assert abs(db) < MOON_TOL["beta_deg"], db
+    assert abs(dr) < MOON_TOL["distance_km"], dr
+
+
+def test_moon_areal_velocity_is_nearly_constant():
+    for t0 in (0.0, 1.0e8, -3.0e8):
+        lo, hi = _areal_velocity_spread(ol.moon, t0)
+        assert 0.98 < lo and hi < 1.02, (t0, lo, hi)
+
+
+@pytest.mark.parametrize("series,term", [("lam", 0), ("lam", 2), ("lam", 3), ("r", 1), ("r", 2),
+                                         ("r", 3), ("beta", 0), ("beta", 1)])
+def test_planted_lunar_sign_flip_is_caught(series, term):
+    """Flipping the sign of a main term of the series (the 22,640", 4,586" evection and 2,370"
+    variation terms in longitude, the main distance and latitude terms) must fail the pins."""
+    lam, beta, r = ol.moon_series_default()
+    amp = {"lam": lam, "beta": beta, "r": r}[series]
+    assert amp[term] != 0.0
+    amp[term] = -amp[term]
+    bad = lambda t: ol.moon_with(t, lam, beta, r)
+    caught = not _meeus_ok(bad)
+    if series == "lam" and term in (0, 2, 3):
+        lo, hi = _areal_velocity_spread(bad)
+        caught_invariant = not (0.98 < lo and hi < 1.02)
+        assert caught and caught_invariant, (series, term, _meeus_47a_errors(bad), lo, hi)
+    assert caught, (series, term, _meeus_47a_errors(bad))
+    # the default amplitudes pass the same checks
+    assert _meeus_ok(lambda t: ol.moon_with(t, *ol.moon_series_default()))
+
+
 def test_earth_rotation_sidereal_day():
     th0 = ol.lib.or_earth_rotation_angle(0.0)
     assert math.isclose(th0, math.radians(280.46061837504), rel_tol=1e-14)
