@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  See oracle.h
  * for what each function follows and what pins it.  Build: oracle/Makefile
- * (gcc -O2 -ffp-contract=off -fno-fast-math -fopenmp).
+ * (gcc -O3 -march=x86-64-v3 -ffp-contract=off -fno-fast-math -fopenmp).
  */
 #include "oracle_internal.h"
 
@@ -294,49 +294,109 @@ typedef struct {
   uint32_t idx;
 } keyidx;
 
-static int cmp_keyidx(const void *a, const void *b) {
-  const keyidx *x = (const keyidx *)a, *y = (const keyidx *)b;
-  if (x->key != y->key) return x->key < y->key ? -1 : 1;
-  return x->idx < y->idx ? -1 : (x->idx > y->idx);
-}
-
 #define GRID_BIAS 1048576LL /* 2^20: cell coordinates stored in 21 bits each */
 static uint64_t pack_cell(long long ix, long long iy, long long iz) {
   return ((uint64_t)(ix + GRID_BIAS) << 42) | ((uint64_t)(iy + GRID_BIAS) << 21) | (uint64_t)(iz + GRID_BIAS);
 }
 
-void or_pairs_grid(const or_objects *o, const or_functor_cfg *cfg, or_pair *hits, size_t cap, size_t *n,
-                   or_pair *evaded, size_t ecap, size_t *ne, int nthreads) {
-  /* linked cells of side cutoff*(1+1e-7): every pair within the cutoff lies in
-   * neighbouring cells; the pair is evaluated from its lower index only. */
-  *n = 0;
-  *ne = 0;
+static uint64_t splitmix64(uint64_t x);
+
+/* Stable LSD radix sort of (key, idx) records by key, 16 bits per pass.  The input is in
+ * index order, so the result is ordered by (key, idx). */
+static void radix_sort_keyidx(keyidx *a, size_t n) {
+  keyidx *tmp = (keyidx *)malloc(sizeof(keyidx) * (n ? n : 1));
+  size_t *cnt = (size_t *)malloc(sizeof(size_t) * 65536);
+  for (int pass = 0; pass < 4; ++pass) {
+    const int sh = 16 * pass;
+    memset(cnt, 0, sizeof(size_t) * 65536);
+    for (size_t i = 0; i < n; ++i) ++cnt[(a[i].key >> sh) & 0xFFFFu];
+    size_t used = 0;
+    for (size_t b = 0; b < 65536; ++b) used += cnt[b] != 0;
+    if (used <= 1) continue; /* every key has the same digit: the pass keeps the order */
+    size_t run = 0;
+    for (size_t b = 0; b < 65536; ++b) {
+      const size_t c = cnt[b];
+      cnt[b] = run;
+      run += c;
+    }
+    for (size_t i = 0; i < n; ++i) tmp[cnt[(a[i].key >> sh) & 0xFFFFu]++] = a[i];
+    memcpy(a, tmp, sizeof(keyidx) * n);
+  }
+  free(cnt);
+  free(tmp);
+}
+
+/* What the grid scan does with each reported pair. */
+typedef struct {
+  pvec hits, ev;        /* list mode */
+  uint64_t nh, ne;      /* digest mode: counts */
+  uint64_t dh[2], de[2]; /* digest mode: sums of the pair hashes (or_pairset_digest) */
+} grid_acc;
+
+static void grid_record(grid_acc *acc, int digest_only, int kind, const or_pair *pr) {
+  if (kind == 0) return;
+  if (!digest_only) {
+    pvec_push(kind == 1 ? &acc->hits : &acc->ev, pr);
+    return;
+  }
+  const uint64_t h = splitmix64(pr->id_a ^ splitmix64(pr->id_b));
+  const uint64_t h2 = splitmix64(h ^ 0xD1B54A32D192ED03ull);
+  if (kind == 1) {
+    acc->nh += 1;
+    acc->dh[0] += h;
+    acc->dh[1] += h2;
+  } else {
+    acc->ne += 1;
+    acc->de[0] += h;
+    acc->de[1] += h2;
+  }
+}
+
+/* Linked cells of side cutoff*(1+1e-7): every pair within the cutoff lies in neighbouring
+ * cells, and AoSFunctor(i, j) is called for it from its lower index i only, so every
+ * unordered pair is evaluated exactly once (AutoPas Newton3 traversal).  Cells are found
+ * by binary search over the sorted unique cell keys.  Returns one accumulator per thread
+ * (*nt of them), or NULL when a coordinate is beyond the packed range (caller falls back
+ * to the brute force). */
+static grid_acc *grid_scan(const or_objects *o, const or_functor_cfg *cfg, int nthreads, int digest_only, int *nt_out) {
   const size_t N = o->n;
-  if (N < 2) return;
   const double s = cfg->cutoff_km * (1.0 + 1e-7);
+  int nt = 1;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+  *nt_out = nt;
+  if (!(s > 0.0)) return NULL;
   long long *cell = (long long *)malloc(sizeof(long long) * 3 * N);
-  int fallback = !(s > 0.0);
-  for (size_t i = 0; i < N && !fallback; ++i) {
+  keyidx *ki = (keyidx *)malloc(sizeof(keyidx) * N);
+  int fallback = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) reduction(| : fallback)
+#endif
+  for (long long ii = 0; ii < (long long)N; ++ii) {
+    const size_t i = (size_t)ii;
     const double u[3] = {o->rx[i] / s, o->ry[i] / s, o->rz[i] / s};
+    int bad = 0;
     for (int k = 0; k < 3; ++k) {
       if (!(fabs(u[k]) < (double)(GRID_BIAS - 2))) {
-        fallback = 1;
+        bad = 1;
         break;
       }
       cell[3 * i + k] = (long long)floor(u[k]);
     }
+    fallback |= bad;
+    if (!bad) {
+      ki[i].key = pack_cell(cell[3 * i], cell[3 * i + 1], cell[3 * i + 2]);
+      ki[i].idx = (uint32_t)i;
+    }
   }
   if (fallback) {
+    free(ki);
     free(cell);
-    or_pairs_bruteforce(o, cfg, hits, cap, n, evaded, ecap, ne);
-    return;
+    return NULL;
   }
-  keyidx *ki = (keyidx *)malloc(sizeof(keyidx) * N);
-  for (size_t i = 0; i < N; ++i) {
-    ki[i].key = pack_cell(cell[3 * i], cell[3 * i + 1], cell[3 * i + 2]);
-    ki[i].idx = (uint32_t)i;
-  }
-  qsort(ki, N, sizeof(keyidx), cmp_keyidx);
+  radix_sort_keyidx(ki, N);
   /* unique cells and their ranges */
   size_t nu = 0;
   uint64_t *ukey = (uint64_t *)malloc(sizeof(uint64_t) * N);
@@ -349,12 +409,7 @@ void or_pairs_grid(const or_objects *o, const or_functor_cfg *cfg, or_pair *hits
     }
   }
   ubeg[nu] = N;
-  int nt = 1;
-#ifdef _OPENMP
-  if (nthreads > 0) omp_set_num_threads(nthreads);
-  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
-#endif
-  pvec *th = (pvec *)calloc((size_t)nt * 2, sizeof(pvec));
+  grid_acc *acc = (grid_acc *)calloc((size_t)nt, sizeof(grid_acc));
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 256)
 #endif
@@ -381,23 +436,64 @@ void or_pairs_grid(const or_objects *o, const or_functor_cfg *cfg, or_pair *hits
           for (size_t q = ubeg[lo]; q < ubeg[lo + 1]; ++q) {
             const size_t j = ki[q].idx;
             if (j <= i) continue;
-            const int kind = or_collision_functor(o, i, j, cfg, &pr);
-            if (kind == 1) pvec_push(&th[2 * tid], &pr);
-            if (kind == 2) pvec_push(&th[2 * tid + 1], &pr);
+            grid_record(&acc[tid], digest_only, or_collision_functor(o, i, j, cfg, &pr), &pr);
           }
         }
   }
-  for (int t = 0; t < nt; ++t) {
-    for (size_t k = 0; k < th[2 * t].n; ++k) emit(&th[2 * t].v[k], 1, hits, cap, n, evaded, ecap, ne);
-    for (size_t k = 0; k < th[2 * t + 1].n; ++k) emit(&th[2 * t + 1].v[k], 2, hits, cap, n, evaded, ecap, ne);
-    free(th[2 * t].v);
-    free(th[2 * t + 1].v);
-  }
-  free(th);
   free(ubeg);
   free(ukey);
   free(ki);
   free(cell);
+  return acc;
+}
+
+void or_pairs_grid(const or_objects *o, const or_functor_cfg *cfg, or_pair *hits, size_t cap, size_t *n,
+                   or_pair *evaded, size_t ecap, size_t *ne, int nthreads) {
+  *n = 0;
+  *ne = 0;
+  if (o->n < 2) return;
+  int nt = 1;
+  grid_acc *acc = grid_scan(o, cfg, nthreads, 0, &nt);
+  if (!acc) {
+    or_pairs_bruteforce(o, cfg, hits, cap, n, evaded, ecap, ne);
+    return;
+  }
+  for (int t = 0; t < nt; ++t) {
+    for (size_t k = 0; k < acc[t].hits.n; ++k) emit(&acc[t].hits.v[k], 1, hits, cap, n, evaded, ecap, ne);
+    for (size_t k = 0; k < acc[t].ev.n; ++k) emit(&acc[t].ev.v[k], 2, hits, cap, n, evaded, ecap, ne);
+    free(acc[t].hits.v);
+    free(acc[t].ev.v);
+  }
+  free(acc);
+}
+
+void or_pairs_grid_digest(const or_objects *o, const or_functor_cfg *cfg, uint64_t *n, uint64_t digest[2],
+                          uint64_t *ne, uint64_t edigest[2], int nthreads) {
+  *n = *ne = 0;
+  digest[0] = digest[1] = edigest[0] = edigest[1] = 0;
+  if (o->n < 2) return;
+  int nt = 1;
+  grid_acc *acc = grid_scan(o, cfg, nthreads, 1, &nt);
+  if (!acc) {
+    /* beyond the packed cell range: the brute force, hashed the same way */
+    grid_acc one;
+    memset(&one, 0, sizeof(one));
+    or_pair pr;
+    for (size_t i = 0; i < o->n; ++i)
+      for (size_t j = i + 1; j < o->n; ++j) grid_record(&one, 1, or_collision_functor(o, i, j, cfg, &pr), &pr);
+    acc = (grid_acc *)calloc(1, sizeof(grid_acc));
+    *acc = one;
+    nt = 1;
+  }
+  for (int t = 0; t < nt; ++t) {
+    *n += acc[t].nh;
+    *ne += acc[t].ne;
+    for (int k = 0; k < 2; ++k) {
+      digest[k] += acc[t].dh[k];
+      edigest[k] += acc[t].de[k];
+    }
+  }
+  free(acc);
 }
 
 static int cmp_pair(const void *a, const void *b) {

@@ -8,7 +8,8 @@
  *
  * Plain C11, FP64 throughout, compiled with -ffp-contract=off and without
  * fast-math so every operation is a single IEEE-754 rounding, matching the
- * reference build (x86-64 baseline flags, no FMA contraction).
+ * reference build (no FMA contraction).  -O3 -march=x86-64-v3 vectorises without
+ * changing any result: without fast-math the compiler may not reassociate.
  *
  * What each part follows (reference paths relative to the LADDS tree):
  *   or_distance_by_linear_interpolation  src/ladds/utils/DistanceApproximation.cpp:13-58
@@ -149,6 +150,13 @@ void or_pairs_bruteforce(const or_objects *o, const or_functor_cfg *cfg, or_pair
                          or_pair *evaded, size_t ecap, size_t *ne);
 void or_pairs_grid(const or_objects *o, const or_functor_cfg *cfg, or_pair *hits, size_t cap, size_t *n,
                    or_pair *evaded, size_t ecap, size_t *ne, int nthreads);
+
+/* Count and order-independent digest (or_pairset_digest's definition) of the collisions
+ * and of the evasions that or_pairs_grid reports, without storing the pairs: the check
+ * for populations whose pair sets are too large to hold (dense constellation clusters).
+ * Same traversal as or_pairs_grid, so it inherits that function's pins. */
+void or_pairs_grid_digest(const or_objects *o, const or_functor_cfg *cfg, uint64_t *n, uint64_t digest[2],
+                          uint64_t *ne, uint64_t edigest[2], int nthreads);
 
 /* sort pairs by (id_a, id_b, idx_a, idx_b) */
 void or_sort_pairs(or_pair *p, size_t n);

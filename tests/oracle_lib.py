@@ -67,6 +67,9 @@ def _load():
                                     C.POINTER(C.c_size_t), C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
     lib.or_pairs_grid.argtypes = [C.POINTER(Objects), C.POINTER(FunctorCfg), C.c_void_p, C.c_size_t,
                                   C.POINTER(C.c_size_t), C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), C.c_int]
+    lib.or_pairs_grid_digest.argtypes = [C.POINTER(Objects), C.POINTER(FunctorCfg), C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                         C.c_int]
     lib.or_sort_pairs.argtypes = [C.c_void_p, C.c_size_t]
     lib.or_pairset_digest.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_uint64)]
     return lib
@@ -205,6 +208,15 @@ def pairs(objs: ObjectSet, cfg: FunctorCfg, method="grid", nthreads=0):
         lib.or_sort_pairs(hits.ctypes.data, hits.shape[0])
         lib.or_sort_pairs(ev.ctypes.data, ev.shape[0])
         return hits, ev
+
+
+def grid_digest(objs: ObjectSet, cfg: FunctorCfg, nthreads=0):
+    """(collisions, digest, evasions, evasion digest) of the grid scan without storing pairs;
+    digests are (u64, u64) tuples with the definition of ``digest``."""
+    n, ne = C.c_uint64(0), C.c_uint64(0)
+    d, ed = (C.c_uint64 * 2)(), (C.c_uint64 * 2)()
+    lib.or_pairs_grid_digest(C.byref(objs.c), C.byref(cfg), C.byref(n), d, C.byref(ne), ed, nthreads)
+    return n.value, (d[0], d[1]), ne.value, (ed[0], ed[1])
 
 
 def sort_pairs(p: np.ndarray) -> np.ndarray:

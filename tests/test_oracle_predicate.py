@@ -136,6 +136,46 @@ def test_grid_equals_bruteforce(n, box, cutoff, seed):
     assert eb.tobytes() == eg.tobytes()
 
 
+@pytest.mark.parametrize("n,box,cutoff,seed", [(400, 20.0, 1.5, 1), (3000, 60.0, 2.0, 3)])
+def test_grid_digest_equals_digest_of_bruteforce(n, box, cutoff, seed):
+    """The digest-only grid scan (used where pair sets are too large to store) reports the
+    counts and digests of the brute-force pair lists; coordinates straddle zero so the
+    packed cell keys cover negative cells too."""
+    r, v, rng = _random_cloud(n, box, seed, vel=0.5)
+    r -= box / 2
+    radius = rng.uniform(100, 2000, n)
+    act = rng.integers(0, 3, n).astype(np.int32)
+    parent = np.where(rng.random(n) < 0.2, rng.integers(0, 5, n), 2**64 - 1).astype(np.uint64)
+    objs = ol.ObjectSet(r, v, radius, act, rng.permutation(n), parent)
+    cfg = ol.functor_cfg(cutoff, 10.0, 1.0, 500.0, 0.5)
+    hb, eb = ol.pairs(objs, cfg, method="brute")
+    nh, dh, ne, de = ol.grid_digest(objs, cfg, nthreads=4)
+    assert hb.shape[0] > 0 and eb.shape[0] > 0
+    assert (nh, dh, ne, de) == (hb.shape[0], ol.digest(hb), eb.shape[0], ol.digest(eb))
+
+
+def test_grid_digest_of_replica_groups():
+    """Constellation-like replica groups (every member within 50 m of its slot): all pairs
+    of a group are conjunctions, so the count is the closed form sum over groups of
+    m (m - 1) / 2, and a planted change of one id changes the digest."""
+    rng = np.random.default_rng(11)
+    groups, m = 40, 60
+    centres = rng.uniform(-7000, 7000, (groups, 3))
+    r = np.repeat(centres, m, axis=0) + rng.uniform(-0.05, 0.05, (groups * m, 3))
+    v = np.repeat(rng.normal(0, 7.0, (groups, 3)), m, axis=0)
+    n = r.shape[0]
+    ids = np.arange(n)
+    objs = ol.ObjectSet(r, v, np.full(n, 0.1), None, ids)
+    cfg = ol.functor_cfg(5.0, 10.0, threshold_mode=1, threshold_km=0.5)
+    nh, dh, ne, _ = ol.grid_digest(objs, cfg)
+    assert nh == groups * m * (m - 1) // 2 and ne == 0
+    hits, _ = ol.pairs(objs, cfg)
+    assert dh == ol.digest(hits)
+    ids2 = ids.copy()
+    ids2[5] = n + 7
+    assert ol.grid_digest(ol.ObjectSet(r, v, np.full(n, 0.1), None, ids2), cfg)[1] != dh
+
+
 def test_cutoff_only_pairs_equal_distance_definition():
     """Integration-test fixture semantics: radius-scaled threshold large, pairs = |r_i - r_j| <= cutoff."""
     r, v, _ = _random_cloud(300, 10.0, 5)
