@@ -94,6 +94,8 @@ def parse():
     p.add_argument("--clustered-steps", type=int, default=10,
                    help="timed steps of the clustered leg (each ~0.4 s: the workload has ~2.7e9 conjunctions "
                         "per step)")
+    p.add_argument("--no-dropin", action="store_true", help="skip the drop-in adapter leg")
+    p.add_argument("--dropin-steps", type=int, default=50)
     p.add_argument("--forms", default="fp64,split")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default plane-tile shape (profiler runs: the tuner's timings change under "
@@ -184,7 +186,7 @@ def cpu_baseline(pop, steps):
     except (OSError, StopIteration):
         pass
     return {"value": n * steps / el, "unit": "object-steps/s", "cores": threads, "kind": "port",
-            "sample": f"C oracle (OpenMP, -O2, no FMA contraction): {n} objects x {steps} steps of the same "
+            "sample": f"C oracle (OpenMP, -O3 -march=x86-64-v3, no FMA contraction): {n} objects x {steps} steps of the same "
                       f"workload (propagate + linked-cell scan), {el:.1f} s on {threads} host threads",
             "cpu_model": model, "nproc": os.cpu_count(), "threads_available": len(os.sched_getaffinity(0)),
             "single_thread": {"value": n1 * steps1 / el1, "sample": f"{n1} objects x {steps1} steps, {el1:.1f} s"}}
@@ -343,6 +345,40 @@ def clustered_leg(la, steps):
     return out
 
 
+def dropin_leg(steps):
+    """The drop-in path as LADDS drives it: the C++ adapter (include/ladds_amd.hpp) built like
+    Simulation.cpp:122-148, then per iteration YoshidaIntegrator::integrate(false) and a fresh
+    CollisionFunctor through Container::iteratePairwise (Simulation.cpp:297, :211-222), with the
+    reference's radius-scaled threshold.  Timed by tests/cpp/adapter_check's `timeloop`
+    (steady clock around the loop, after a warm-up loop), FP64 form, on the 16,384-object
+    Kepler + J2 workload and the 1,048,576-object full-model workload."""
+    import subprocess
+    import tempfile
+
+    exe = ROOT / "tests" / "cpp" / "adapter_check"
+    if not exe.exists():
+        return {"error": f"{exe} not built"}
+    out = {}
+    for name, n, comps, workload in (("16k_kepler_j2", 16384, "1 1 0 0 0 0 0 0", 0),
+                                     ("1m_full_model", 1 << 20, "1 1 1 1 1 1 1 1", 1)):
+        scen = [f"container 10.0 {0} 0", f"accumulator {comps} 0.0", "integrator 10.0",
+                f"population {workload} {0x1ADD5 + workload} {n} 0",
+                "timeloop 5 10.0 10.0 1.0 0.05 0.1", f"timeloop {steps} 10.0 10.0 1.0 0.05 0.1"]
+        with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
+            f.write("\n".join(scen) + "\n")
+            path = f.name
+        res = subprocess.run([str(exe), path], capture_output=True, text=True, timeout=600)
+        os.unlink(path)
+        if res.returncode != 0:
+            out[name] = {"error": res.stderr[-500:]}
+            continue
+        t = [ln.split() for ln in res.stdout.splitlines() if ln.startswith("timing ")][-1]
+        sec = float(t[2])
+        out[name] = {"objects": n, "steps": steps, "object_steps_per_s": n * steps / sec,
+                     "ms_per_step": 1e3 * sec / steps, "collisions": int(t[3]), "evasions": int(t[4])}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -445,6 +481,9 @@ def main():
     if not a.no_clustered:
         out["clustered"] = clustered_leg(la, a.clustered_steps)
         print(f"# clustered: {json.dumps(out['clustered'])}", file=sys.stderr, flush=True)
+    if not a.no_dropin:
+        out["dropin"] = dropin_leg(a.dropin_steps)
+        print(f"# dropin: {json.dumps(out['dropin'])}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pop, a.cpu_steps)
         if grads is not None:

@@ -205,7 +205,9 @@ LADDS_API int ladds_get_ids(ladds_ctx *ctx, uint64_t *id);
 LADDS_API double ladds_get_time(const ladds_ctx *ctx);
 LADDS_API int ladds_set_time(ladds_ctx *ctx, double t);
 
-/* YoshidaIntegrator::integrate(false), nsteps times (Simulation.cpp:297). */
+/* YoshidaIntegrator::integrate(false), nsteps times (Simulation.cpp:297).  Enqueues the
+ * steps on the context's stream and returns; every call that returns data to the host
+ * (ladds_get_state, ladds_conjunctions, ...) waits for them. */
 LADDS_API int ladds_propagate(ladds_ctx *ctx, int nsteps);
 
 /* Simulation::collisionDetection (Simulation.cpp:211-222): AutoPas::iteratePairwise with a
@@ -213,6 +215,11 @@ LADDS_API int ladds_propagate(ladds_ctx *ctx, int nsteps);
  * state.  Unordered.  On overflow returns LADDS_E_OVERFLOW with *n / *ne = required counts. */
 LADDS_API int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_t cap, size_t *n,
                        ladds_pair *evaded, size_t ecap, size_t *ne);
+
+/* State of the objects at device indices idx[0..count) (ladds_pair.idx_a / idx_b), as
+ * r[3 * k + c] and v[3 * k + c] -- what CollisionFunctor's (Particle*, Particle*, ...) tuples
+ * point at (CollisionFunctor.h:95-108), without downloading the whole population. */
+LADDS_API int ladds_gather_state(ladds_ctx *ctx, const uint32_t *idx, size_t count, double *r, double *v);
 
 /* Count + order-independent 128-bit digest of the collision and evasion sets on the current
  * state, without materialising pairs (for populations whose pair sets do not fit memory). */
@@ -223,9 +230,17 @@ LADDS_API int ladds_conjunction_digest(ladds_ctx *ctx, double dt_interp, uint64_
  * [first_iteration, first_iteration + nsteps): integrate; deleteBurnUps if
  * min_altitude_km > 0 (Simulation.cpp:301-303); collision detection when
  * iteration % tpcd == 0 with window dt*tpcd (Simulation.cpp:389-397).  Pairs are
- * appended to the device log (ladds_fetch_log); totals in *stats (may be NULL). */
+ * appended to the device log (ladds_fetch_log); totals in *stats (may be NULL).
+ * Spawn protection as the loop keeps it (Simulation.cpp:287-293, 399-402, 438): the
+ * context counts iterations since the last one with collisions (from 0 at
+ * ladds_create, continued across calls and uploads); at the start of an iteration where the
+ * count is 100, every parent id is reset (removeParticleSpawnProtection, :470-473). */
 LADDS_API int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpcd, double min_altitude_km,
                    ladds_run_stats *stats);
+/* Simulation::removeParticleSpawnProtection (Simulation.cpp:470-473): every parent id set
+ * to "none" (UINT64_MAX), so fragments of one breakup can conjunct again -- for callers
+ * that drive the loop themselves (integrate + ladds_conjunctions). */
+LADDS_API int ladds_remove_spawn_protection(ladds_ctx *ctx);
 LADDS_API int ladds_fetch_log(ladds_ctx *ctx, ladds_logged_pair *out, size_t cap, size_t *n);
 LADDS_API int ladds_clear_log(ladds_ctx *ctx);
 

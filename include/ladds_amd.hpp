@@ -24,6 +24,7 @@
 #ifndef LADDS_AMD_HPP
 #define LADDS_AMD_HPP
 
+#include <algorithm>
 #include <array>
 #include <charconv>
 #include <cmath>
@@ -87,6 +88,7 @@ class Particle {
   double getBcInv() const { return bc_inv_; }
   double getAom() const { return aom_; }
   size_t getParentIdentifier() const { return parent_; }
+  void setParentIdentifier(size_t parent) { parent_ = parent; }
   double getCr() const { return cr_; }
   double getHeight() const { return std::sqrt(r_[0] * r_[0] + r_[1] * r_[1] + r_[2] * r_[2]); }
   double getSpeed() const { return std::sqrt(v_[0] * v_[0] + v_[1] * v_[1] + v_[2] * v_[2]); }
@@ -128,6 +130,8 @@ class Container {
     cfg_.scan_variant = c.scanVariant;
     cfg_.device = c.device;
     check(ladds_create(&cfg_, &ctx_), nullptr, "ladds_create");
+    applied_dt_ = cfg_.dt;
+    applied_t0_ = cfg_.t0;
   }
   ~Container() { ladds_destroy(ctx_); }
   Container(const Container &) = delete;
@@ -148,6 +152,21 @@ class Container {
     host_stale_ = true;
   }
   size_t getNumberOfParticles() { syncHost(); return particles_.size(); }
+  /* autopas.forEach: f applied to every object of the host mirror, uploaded before the
+   * next device call. */
+  template <typename F>
+  void forEach(F f) {
+    syncHost();
+    for (Particle &p : particles_) f(p);
+    device_dirty_ = true;
+  }
+  /* Simulation::removeParticleSpawnProtection (Simulation.cpp:470-473) on the device: every
+   * parent id reset, without an upload of the population. */
+  void removeParticleSpawnProtection() {
+    syncDevice();
+    check(ladds_remove_spawn_protection(ctx_), ctx_, "ladds_remove_spawn_protection");
+    for (Particle &p : particles_) p.setParentIdentifier(std::numeric_limits<size_t>::max());
+  }
   /* Host view of the objects; reading it after device work downloads the state first. */
   iterator begin() { syncHost(); device_dirty_ = true; return particles_.begin(); }
   iterator end() { return particles_.end(); }
@@ -213,16 +232,26 @@ class Container {
   /* ---- used by the adapter classes ---- */
   ladds_ctx *context() { return ctx_; }
   ladds_config &config() { return cfg_; }
-  void applyConfig() { check(ladds_configure(ctx_, &cfg_), ctx_, "ladds_configure"); }
+  void applyConfig() {
+    const bool dt_only = cfg_.dt != applied_dt_ && cfg_.t0 == applied_t0_;
+    check(ladds_configure(ctx_, &cfg_), ctx_, "ladds_configure");
+    // a new step alone keeps the elapsed time: the context's clock restarts at the current time
+    if (dt_only) cfg_.t0 = ladds_get_time(ctx_);
+    applied_dt_ = cfg_.dt;
+    applied_t0_ = cfg_.t0;
+  }
   /* Upload the host mirror if it was changed since the last upload: the whole mirror after
    * host-side edits, only the new tail after addParticle calls (breakup fragments). */
   void syncDevice() {
     const size_t n = particles_.size();
-    if (device_dirty_) upload(0, n, ladds_set_objects, "ladds_set_objects");
-    else if (appended_) upload(n - appended_, n, ladds_add_objects, "ladds_add_objects");
+    if (device_dirty_) {
+      upload(0, n, ladds_set_objects, "ladds_set_objects");
+      host_stale_ = false;  // the device now holds exactly the mirror
+    } else if (appended_) {
+      upload(n - appended_, n, ladds_add_objects, "ladds_add_objects");
+    }
     device_dirty_ = false;
     appended_ = 0;
-    host_stale_ = false;
   }
   /* Download the device state into the host mirror if the device moved ahead of it.
    * Objects removed on the device (burn-ups) are removed from the mirror, order kept. */
@@ -250,6 +279,22 @@ class Container {
     host_stale_ = false;
   }
   void markDeviceAhead() { host_stale_ = true; }
+  /* Bring the mirror entries of the given device indices up to date (the objects a
+   * detection returned), or the whole mirror when objects were removed on the device. */
+  void refresh(const std::vector<uint32_t> &idx) {
+    if (!host_stale_) return;
+    if (ladds_num_objects(ctx_) != particles_.size()) {
+      syncHost();
+      return;
+    }
+    if (idx.empty()) return;
+    std::vector<double> r(3 * idx.size()), v(3 * idx.size());
+    check(ladds_gather_state(ctx_, idx.data(), idx.size(), r.data(), v.data()), ctx_, "ladds_gather_state");
+    for (size_t k = 0; k < idx.size(); ++k) {
+      particles_[idx[k]].setPosition({r[3 * k], r[3 * k + 1], r[3 * k + 2]});
+      particles_[idx[k]].setVelocity({v[3 * k], v[3 * k + 1], v[3 * k + 2]});
+    }
+  }
   /* Particle* of the object at device index idx (host mirror must be current). */
   Particle *particleAt(uint32_t idx) { return &particles_.at(idx); }
 
@@ -261,6 +306,7 @@ class Container {
   bool device_dirty_ = false;
   bool host_stale_ = false;
   size_t appended_ = 0;
+  double applied_dt_ = -1.0, applied_t0_ = 0.0;  // the configuration the context holds
 
   void upload(size_t b, size_t e, int (*entry)(ladds_ctx *, const ladds_objects *), const char *what) {
     const size_t n = e - b;
@@ -373,7 +419,18 @@ inline bool Container::iteratePairwise(CollisionFunctor *f) {
   check(rc, ctx_, "ladds_conjunctions");
   hits.resize(n);
   evaded.resize(ne);
-  syncHost();
+  // only the objects the tuples point at are brought up to date (the rest of the mirror
+  // stays behind the device until it is read)
+  std::vector<uint32_t> touched;
+  touched.reserve(2 * (n + ne));
+  for (const auto *list : {&hits, &evaded})
+    for (const ladds_pair &p : *list) {
+      touched.push_back(p.idx_a);
+      touched.push_back(p.idx_b);
+    }
+  std::sort(touched.begin(), touched.end());
+  touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+  refresh(touched);
   auto convert = [this](const std::vector<ladds_pair> &src, CollisionFunctor::CollisionCollectionT &dst) {
     dst.clear();
     dst.reserve(src.size());

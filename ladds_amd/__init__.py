@@ -274,6 +274,9 @@ class Engine:
             setattr(self.config, k, v)
         self._c = self.config.to_c()
         self._check(lib.ladds_configure(self._h, C.byref(self._c)), "configure")
+        if "dt" in changes and "t0" not in changes:
+            # a new step alone keeps the elapsed time: the context's clock restarts at the current time
+            self.config.t0 = self.time
 
     # -- state ----------------------------------------------------------
     @staticmethod
@@ -323,6 +326,7 @@ class Engine:
 
     def set_time(self, t: float):
         self._check(lib.ladds_set_time(self._h, t), "set_time")
+        self.config.t0 = float(t)  # later configure() calls keep this clock
 
     def state(self):
         """(r, v) as (n, 3) arrays."""
@@ -373,6 +377,19 @@ class Engine:
         return {"collisions": st.collisions, "evasions": st.evasions, "logged": st.logged,
                 "candidates": st.candidates, "digest": (st.digest[0], st.digest[1]),
                 "edigest": (st.edigest[0], st.edigest[1]), "remaining": st.remaining}
+
+    def gather_state(self, idx):
+        """(r, v) of the objects at the given device indices, as (k, 3) arrays."""
+        idx = np.ascontiguousarray(idx, np.uint32)
+        r = np.empty((idx.size, 3))
+        v = np.empty((idx.size, 3))
+        self._check(lib.ladds_gather_state(self._h, _ptr(idx, C.c_uint32), idx.size, _ptr(r), _ptr(v)),
+                    "gather_state")
+        return r, v
+
+    def remove_spawn_protection(self):
+        """``Simulation::removeParticleSpawnProtection``: every parent id reset to none."""
+        self._check(lib.ladds_remove_spawn_protection(self._h), "remove_spawn_protection")
 
     def fetch_log(self) -> np.ndarray:
         n = C.c_size_t(0)

@@ -168,6 +168,9 @@ _SIGNATURES = {
     "ladds_conjunction_digest": (C.c_int, [_ctx, C.c_double, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "ladds_simulate": (C.c_int, [_ctx, C.c_uint64, C.c_int, C.c_int, C.c_double, C.POINTER(RunStats)]),
+    "ladds_remove_spawn_protection": (C.c_int, [_ctx]),
+    "ladds_gather_state": (C.c_int, [_ctx, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
     "ladds_fetch_log": (C.c_int, [_ctx, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "ladds_clear_log": (C.c_int, [_ctx]),
     "ladds_delete_burnups": (C.c_int, [_ctx, C.c_double, C.POINTER(C.c_size_t)]),

@@ -41,6 +41,12 @@ constexpr const char *kKernelNames[] = {"propagate",   "bin_count", "bin_reduce"
                                         "bin_scatter", "pair_scan", "bin_keys",       "tile_partition",
                                         "pair_tiles",  "compact"};
 
+struct HostReadback {
+  ScanCounters sc;
+  unsigned long long log_count;
+  SpawnState spawn;
+};
+
 // Pair-scan pipelines (ladds_config.scan_variant).
 enum : int { kVariantAuto = 0, kVariantAtomicSort = 1, kVariantPlaneTiles = 2 };
 
@@ -83,18 +89,35 @@ struct ladds_ctx {
   uint32_t grid_buckets_alloc = 0;
   size_t tile_sum_alloc = 0;
 
+  // Environments of one segment of up to kEnvSegment steps, evaluated on the device by
+  // k_substep_env in stream order before the steps that read them.
   SubstepEnv<double> *env64 = nullptr;
   SubstepEnv<float> *env32 = nullptr;
-  // Environments of one segment of up to kEnvSegment steps: computed on the host
-  // into pinned memory and copied asynchronously, so the host prepares chunk k+1
-  // while the device runs chunk k.
-  SubstepEnv<double> *h_env64 = nullptr;
-  SubstepEnv<float> *h_env32 = nullptr;
 
   LoggedPair *log = nullptr;
   uint64_t log_cap = 0;
   unsigned long long *log_count = nullptr;
   ScanCounters *counters = nullptr;
+  // Pinned host copies of the counters, the log count and the spawn state: one
+  // asynchronous read-back and one synchronisation at the end of ladds_simulate.
+  HostReadback *h_rb = nullptr;
+  // Pinned staging for pair-log entries returned by ladds_conjunctions, and for the
+  // object states gathered by ladds_gather_state.
+  LoggedPair *h_log = nullptr;
+  size_t h_log_cap = 0;
+  void *h_x = nullptr;
+  size_t h_x_bytes = 0;
+  // The device pair log may hold entries of ladds_simulate not yet cleared.
+  bool log_nonempty = false;
+
+  // Spawn protection (Simulation.cpp:287-293): `quiet` is the count of iterations since the
+  // last collision at the start of the next iteration to run.  While objects with a parent
+  // id exist (`protected_objects`), ladds_simulate keeps the count on the device
+  // (`d_spawn`) and resets the parent ids there; otherwise the count is kept on the host
+  // from the latest iteration with collisions.
+  uint64_t quiet = 0;  // from 0 at ladds_create, as Simulation.h:227; continued across calls
+  bool protected_objects = false;
+  SpawnState *d_spawn = nullptr;
 
   // Ensemble scenarios (ladds_set_scenarios): offsets of S scenarios, S + 1 entries; empty
   // for a single population.  Device tile tables of the plane-tile pipeline.
@@ -118,6 +141,7 @@ struct ladds_ctx {
   // tile offsets and the scratch arena are double-buffered; buffer 0 is `d` itself.
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_part[2] = {nullptr, nullptr}, ev_pair[2] = {nullptr, nullptr};
+  hipEvent_t ev_join = nullptr;  // end of ladds_simulate: stream waits for stream2
   double *sb_r[3] = {nullptr, nullptr, nullptr}, *sb_v[3] = {nullptr, nullptr, nullptr};
   int32_t *sb_act = nullptr;
   size_t sb_cap = 0;
@@ -390,6 +414,11 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   // capacity are sorted in the global scratch arena.
   {
     const size_t hdr = 4 * (size_t)tp.lds_header_words;
+    // the header (segment tables of every tile, bucket cursors) grows with the tile count:
+    // it must leave room for a workgroup's worth of records
+    if (tiles && hdr + 256 * rec + kLdsMargin > kPairTileLdsBytes)
+      return ctx->fail(LADDS_E_INVALID, "too many partition tiles or buckets per plane for the pair kernel's LDS: "
+                                        "use fewer objects per context or a larger partition tile");
     const size_t fit = (kPairTileLdsBytes - hdr) / rec;
     const size_t need = std::min(fit, std::max<size_t>(1024, (pair_max + 63) / 64 * 64));
     const size_t wave_cap = std::max<size_t>(1, 16 / std::max<uint32_t>(1, tp.pair_threads / 64));
@@ -437,32 +466,23 @@ FunctorParams functor_params(const ladds_ctx *ctx, double dt_interp) {
 }
 
 constexpr int kEnvSegment = 8192;  // steps per environment segment
-constexpr int kChunk = 1024;       // steps per host-side preparation chunk
+constexpr int kChunk = 1024;       // steps per propagation launch of ladds_propagate
 
 int alloc_env(ladds_ctx *ctx) {
   if (ctx->env64) return LADDS_OK;
   const size_t need = 3 * (size_t)kEnvSegment;
   HIPCHK(ctx, dalloc(&ctx->env64, need));
   HIPCHK(ctx, dalloc(&ctx->env32, need));
-  HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&ctx->h_env64), sizeof(SubstepEnv<double>) * need, 0));
-  HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&ctx->h_env32), sizeof(SubstepEnv<float>) * need, 0));
   return LADDS_OK;
 }
 
 // Environments of steps [first, first + count) of the segment starting at the context's
-// current step; enqueues their upload.  Callers synchronise before a new segment reuses
-// the pinned buffer.
+// current step, evaluated on the context's stream (ordered before the propagation that
+// reads them and after the one that read the previous segment's).
 int stage_env(ladds_ctx *ctx, int first, int count) {
   const double tau[3] = {ctx->c[0], ctx->c[0] + ctx->c[1], ctx->c[0] + ctx->c[1] + ctx->c[2]};
-  for (int s = first; s < first + count; ++s) {
-    const double t = ctx->cfg.t0 + (double)(ctx->step_count + (uint64_t)s) * ctx->cfg.dt;
-    for (int k = 0; k < 3; ++k) substep_env(t + tau[k] * ctx->cfg.dt, ctx->h_env64[3 * s + k], ctx->h_env32[3 * s + k]);
-  }
-  const size_t off = 3 * (size_t)first, cnt = 3 * (size_t)count;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->env64 + off, ctx->h_env64 + off, sizeof(SubstepEnv<double>) * cnt,
-                             hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->env32 + off, ctx->h_env32 + off, sizeof(SubstepEnv<float>) * cnt,
-                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, launch_substep_env(ctx->cfg.t0, ctx->cfg.dt, ctx->step_count, tau, first, count, ctx->env64,
+                                 ctx->env32, ctx->stream));
   return LADDS_OK;
 }
 
@@ -490,17 +510,26 @@ int launch_propagate_steps(ladds_ctx *ctx, size_t env_offset, int nsteps, bool b
                             ctx->cfg.components, cdt, ddt, &ctx->grid,
                             (bin && ctx->variant == kVariantPlaneTiles) ? ctx->slot : nullptr,
                             (bin && ctx->variant == kVariantAtomicSort) ? ctx->count : nullptr, ctx->slot, burnup_r2,
-                            step_state, ctx->stream);
+                            step_state, ctx->stream, nullptr, 0, &ctx->counters->burnups);
   });
   if (e != hipSuccess) return ctx->hipfail(e, "propagate launch");
   return LADDS_OK;
 }
 
+// Spawn-protection bookkeeping around the pair scan of a detection that closes `span`
+// iterations (0: none, no objects with a parent id).
+hipError_t spawn_pre(ladds_ctx *ctx, uint64_t span, hipStream_t stream) {
+  return span ? launch_clear_parents(ctx->d_spawn, span, ctx->d.parent, ctx->n, stream) : hipSuccess;
+}
+hipError_t spawn_post(ladds_ctx *ctx, uint64_t span, bool detect, hipStream_t stream) {
+  return span ? launch_spawn_post(ctx->d_spawn, span, ctx->counters, detect, stream) : hipSuccess;
+}
+
 // Binning (after the propagation's binning pass when `binned`) + pair scan on the current state.
 int launch_detection(ladds_ctx *ctx, bool binned, double dt_interp, bool digest, uint64_t iteration,
-                     const StepState *step_state = nullptr) {
+                     const StepState *step_state = nullptr, uint64_t spawn_span = 0) {
   const FunctorParams fp = functor_params(ctx, dt_interp);
-  hipError_t e = hipSuccess;
+  hipError_t e = spawn_pre(ctx, spawn_span, ctx->stream);
   if (ctx->variant == kVariantPlaneTiles) {
     if (!binned)
       e = timed(ctx, "bin_keys", [&] { return launch_bin_keys(ctx->form, ctx->d, ctx->grid, ctx->slot, ctx->stream); });
@@ -515,6 +544,7 @@ int launch_detection(ladds_ctx *ctx, bool binned, double dt_interp, bool digest,
                                  ctx->scratch, ctx->arena_top, ctx->log, ctx->log_cap, ctx->log_count, ctx->counters,
                                  iteration, step_state, ctx->stream);
       });
+    if (e == hipSuccess) e = spawn_post(ctx, spawn_span, true, ctx->stream);
     if (e != hipSuccess) return ctx->hipfail(e, "detection launch");
     return LADDS_OK;
   }
@@ -535,6 +565,7 @@ int launch_detection(ladds_ctx *ctx, bool binned, double dt_interp, bool digest,
       return launch_pair_scan(ctx->form, digest, ctx->d, ctx->grid, fp, ctx->rec, ctx->start, ctx->log,
                               ctx->log_cap, ctx->log_count, ctx->counters, iteration, ctx->stream);
     });
+  if (e == hipSuccess) e = spawn_post(ctx, spawn_span, true, ctx->stream);
   if (e != hipSuccess) return ctx->hipfail(e, "detection launch");
   return LADDS_OK;
 }
@@ -608,7 +639,8 @@ int launch_prop_part(ladds_ctx *ctx, int cur, size_t env_offset, int nsteps, boo
   hipError_t e = timed(ctx, "propagate", [&] {
     return launch_propagate(ctx->form, in, ctx->env64 + 3 * env_offset, ctx->env32 + 3 * env_offset, nsteps,
                             ctx->cfg.components, cdt, ddt, &ctx->grid, partition ? ctx->slot : nullptr, nullptr,
-                            ctx->slot, burnup_r2, step_state, ctx->stream, &out, step_offset);
+                            ctx->slot, burnup_r2, step_state, ctx->stream, &out, step_offset,
+                            &ctx->counters->burnups);
   });
   if (e == hipSuccess && partition)
     e = timed(ctx, "tile_partition", [&] {
@@ -619,16 +651,20 @@ int launch_prop_part(ladds_ctx *ctx, int cur, size_t env_offset, int nsteps, boo
   return LADDS_OK;
 }
 
-// Pair scan of state buffer b (records partitioned into buffer b) on `stream`.
+// Pair scan of state buffer b (records partitioned into buffer b) on `stream`, with the
+// spawn-protection bookkeeping of `spawn_span` iterations around it.
 int launch_pairs_of(ladds_ctx *ctx, int b, double dt_interp, bool digest, uint64_t iteration,
-                    const StepState *step_state, hipStream_t stream) {
+                    const StepState *step_state, hipStream_t stream, uint64_t spawn_span = 0) {
   const FunctorParams fp = functor_params(ctx, dt_interp);
-  const hipError_t e = timed_on(ctx, "pair_tiles", stream, [&] {
-    return launch_pair_tiles(ctx->form, digest, state_view(ctx, b), ctx->grid, ctx->tiles, fp,
-                             b ? ctx->rec2 : ctx->rec, b ? ctx->tps2 : ctx->tps, b ? ctx->scratch2 : ctx->scratch,
-                             b ? ctx->arena_top2 : ctx->arena_top, ctx->log, ctx->log_cap, ctx->log_count,
-                             ctx->counters, iteration, step_state, stream);
-  });
+  hipError_t e = spawn_pre(ctx, spawn_span, stream);
+  if (e == hipSuccess)
+    e = timed_on(ctx, "pair_tiles", stream, [&] {
+      return launch_pair_tiles(ctx->form, digest, state_view(ctx, b), ctx->grid, ctx->tiles, fp,
+                               b ? ctx->rec2 : ctx->rec, b ? ctx->tps2 : ctx->tps, b ? ctx->scratch2 : ctx->scratch,
+                               b ? ctx->arena_top2 : ctx->arena_top, ctx->log, ctx->log_cap, ctx->log_count,
+                               ctx->counters, iteration, step_state, stream);
+    });
+  if (e == hipSuccess) e = spawn_post(ctx, spawn_span, true, stream);
   if (e != hipSuccess) return ctx->hipfail(e, "pair scan launch");
   return LADDS_OK;
 }
@@ -636,7 +672,7 @@ int launch_pairs_of(ladds_ctx *ctx, int b, double dt_interp, bool digest, uint64
 // One overlapped iteration: propagate from buffer `cur` into buffer 1 - cur (stream),
 // and when `detect`, partition it (stream) and scan its pairs (stream2).
 int overlapped_iteration(ladds_ctx *ctx, int cur, bool pending_next, size_t env_offset, int nsteps, bool detect,
-                         double burnup_r2, double dt_interp, bool digest, uint64_t iteration) {
+                         double burnup_r2, double dt_interp, bool digest, uint64_t iteration, uint64_t spawn_span) {
   const int nxt = 1 - cur;
   // buffer nxt was last read by a pair scan on stream2
   if (pending_next) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_pair[nxt], 0));
@@ -644,7 +680,7 @@ int overlapped_iteration(ladds_ctx *ctx, int cur, bool pending_next, size_t env_
   if (rc != LADDS_OK || !detect) return rc;
   HIPCHK(ctx, hipEventRecord(ctx->ev_part[nxt], ctx->stream));
   HIPCHK(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_part[nxt], 0));
-  rc = launch_pairs_of(ctx, nxt, dt_interp, digest, iteration, nullptr, ctx->stream2);
+  rc = launch_pairs_of(ctx, nxt, dt_interp, digest, iteration, nullptr, ctx->stream2, spawn_span);
   if (rc != LADDS_OK) return rc;
   HIPCHK(ctx, hipEventRecord(ctx->ev_pair[nxt], ctx->stream2));
   return LADDS_OK;
@@ -661,7 +697,8 @@ int capture_step_graph(ladds_ctx *ctx, int b, double burnup_r2, bool digest, hip
   int rc = LADDS_OK;
   hipError_t e = hipEventRecord(ctx->ev_part[0], ctx->stream);  // fork
   if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream2, ctx->ev_part[0], 0);
-  if (e == hipSuccess) rc = launch_pairs_of(ctx, b, ctx->cfg.dt, digest, 0, ctx->d_step, ctx->stream2);
+  if (e == hipSuccess)
+    rc = launch_pairs_of(ctx, b, ctx->cfg.dt, digest, 0, ctx->d_step, ctx->stream2, ctx->protected_objects ? 1 : 0);
   if (e == hipSuccess && rc == LADDS_OK) rc = launch_prop_part(ctx, b, 0, 1, true, burnup_r2, ctx->d_step, 1u);
   if (e == hipSuccess && rc == LADDS_OK) e = hipEventRecord(ctx->ev_pair[0], ctx->stream2);  // join
   if (e == hipSuccess && rc == LADDS_OK) e = hipStreamWaitEvent(ctx->stream, ctx->ev_pair[0], 0);
@@ -779,6 +816,8 @@ int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
     e = hipEventCreateWithFlags(&ctx->ev_part[b], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_pair[b], hipEventDisableTiming);
   }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_rb), sizeof(HostReadback), 0);
   if (e == hipSuccess) e = dalloc(&ctx->arena_top2, 1);
   if (e == hipSuccess) e = upload_atmosphere();
   if (e == hipSuccess) e = configure_tile_kernels();
@@ -819,6 +858,11 @@ int ladds_destroy(ladds_ctx *ctx) {
     if (ctx->ev_part[b]) hipEventDestroy(ctx->ev_part[b]);
     if (ctx->ev_pair[b]) hipEventDestroy(ctx->ev_pair[b]);
   }
+  if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+  if (ctx->h_rb) hipHostFree(ctx->h_rb);
+  if (ctx->h_log) hipHostFree(ctx->h_log);
+  if (ctx->h_x) hipHostFree(ctx->h_x);
+  hipFree(ctx->d_spawn);
   if (ctx->stream2) hipStreamDestroy(ctx->stream2);
   free_objects(ctx->d);
   free_objects(ctx->d_alt);
@@ -833,8 +877,6 @@ int ladds_destroy(ladds_ctx *ctx) {
   hipFree(ctx->arena_top);
   hipFree(ctx->env64);
   hipFree(ctx->env32);
-  if (ctx->h_env64) hipHostFree(ctx->h_env64);
-  if (ctx->h_env32) hipHostFree(ctx->h_env32);
   hipFree(ctx->log);
   hipFree(ctx->log_count);
   hipFree(ctx->counters);
@@ -874,9 +916,18 @@ int ladds_configure(ladds_ctx *ctx, const ladds_config *cfg) {
     ctx->form = cfg->precision;
     ctx->variant = (cfg->scan_variant == kVariantAtomicSort) ? kVariantAtomicSort : kVariantPlaneTiles;
   }
-  if (cfg->t0 != old.t0 || cfg->dt != old.dt) ctx->step_count = 0;
+  // A new t0 restarts the clock there; a new dt alone keeps the elapsed time (the clock
+  // continues from the current time with the new step).
+  const double t_now = ctx->time();
+  const bool new_t0 = cfg->t0 != old.t0, new_dt = cfg->dt != old.dt;
   const uint64_t log_cap = cfg->log_capacity;
   ctx->cfg = *cfg;
+  if (new_t0) {
+    ctx->step_count = 0;
+  } else if (new_dt) {
+    ctx->cfg.t0 = t_now;
+    ctx->step_count = 0;
+  }
   HIPCHK(ctx, hipSetDevice(ctx->device));
   return ensure_log(ctx, std::max<uint64_t>(log_cap, 1024));
 }
@@ -884,6 +935,13 @@ int ladds_configure(ladds_ctx *ctx, const ladds_config *cfg) {
 }  // extern "C"
 
 namespace {
+
+bool has_parents(const ladds_objects *o) {
+  if (!o->parent) return false;
+  for (size_t i = 0; i < o->n; ++i)
+    if (o->parent[i] != UINT64_MAX) return true;
+  return false;
+}
 
 int check_objects(ladds_ctx *ctx, const ladds_objects *o) {
   if (o->n > 0 && (!o->rx || !o->ry || !o->rz || !o->vx || !o->vy || !o->vz || !o->radius_m))
@@ -1016,6 +1074,7 @@ int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
   if (rc != LADDS_OK) return rc;
   ctx->n = n;
   ctx->d.n = n;
+  ctx->protected_objects = has_parents(o);
   return upload_objects(ctx, o, 0);
 }
 
@@ -1034,6 +1093,7 @@ int ladds_add_objects(ladds_ctx *ctx, const ladds_objects *o) {
   if (rc != LADDS_OK) return rc;
   ctx->n = n;
   ctx->d.n = n;
+  ctx->protected_objects = ctx->protected_objects || has_parents(o);
   std::vector<double> rx(n);
   HIPCHK(ctx, hipMemcpy(rx.data(), ctx->d.rx, sizeof(double) * n, hipMemcpyDeviceToHost));
   return setup_grid(ctx, n, rx.data());
@@ -1118,13 +1178,10 @@ int ladds_propagate(ladds_ctx *ctx, int nsteps) {
   }
   for (int seg0 = 0; seg0 < nsteps; seg0 += kEnvSegment) {
     const int seg = std::min(kEnvSegment, nsteps - seg0);
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // pinned environment buffer is free again
-    for (int s = 0; s < seg; s += kChunk) {
-      const int k = std::min(kChunk, seg - s);
-      int rc = stage_env(ctx, s, k);
-      if (rc == LADDS_OK) rc = launch_propagate_steps(ctx, (size_t)s, k, false, 0.0);
-      if (rc != LADDS_OK) return rc;
-    }
+    int rc = stage_env(ctx, 0, seg);
+    for (int s = 0; s < seg && rc == LADDS_OK; s += kChunk)
+      rc = launch_propagate_steps(ctx, (size_t)s, std::min(kChunk, seg - s), false, 0.0);
+    if (rc != LADDS_OK) return rc;
     ctx->step_count += (uint64_t)seg;
   }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1174,6 +1231,9 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   if (ctx->n < 2) return ctx->fail(LADDS_E_STATE, "set objects before autotuning");
   if (ctx->variant != kVariantPlaneTiles) return ctx->fail(LADDS_E_STATE, "autotuning covers the plane-tile scan");
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  // the tuner's detections share the device pair log with ladds_simulate: refuse to drop
+  // an unfetched log
+  if (ctx->log_nonempty) return ctx->fail(LADDS_E_STATE, "pair log not empty: fetch and clear it before ladds_autotune");
   const size_t n = ctx->n;
   std::vector<double> rx(n);
   HIPCHK(ctx, hipMemcpy(rx.data(), ctx->d.rx, sizeof(double) * n, hipMemcpyDeviceToHost));
@@ -1276,6 +1336,25 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   return LADDS_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+int ensure_xbuf(ladds_ctx *ctx, size_t bytes) {
+  if (!ctx->xcount) HIPCHK(ctx, dalloc(&ctx->xcount, 2));
+  if (bytes <= ctx->xbuf_bytes) return LADDS_OK;
+  hipFree(ctx->xbuf);
+  ctx->xbuf = nullptr;
+  ctx->xbuf_bytes = 0;
+  HIPCHK(ctx, hipMalloc(&ctx->xbuf, bytes));
+  ctx->xbuf_bytes = bytes;
+  return LADDS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_t cap, size_t *n, ladds_pair *evaded,
                        size_t ecap, size_t *ne) {
   if (!ctx || !n || !ne || (cap && !hits) || (ecap && !evaded)) return LADDS_E_INVALID;
@@ -1283,26 +1362,51 @@ int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_
   if (ctx->n < 2) return LADDS_OK;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   // the device log is shared with ladds_simulate: preserve its contents
-  unsigned long long saved = 0;
-  HIPCHK(ctx, hipMemcpyAsync(&saved, ctx->log_count, sizeof(saved), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  if (saved) return ctx->fail(LADDS_E_STATE, "pair log not empty: fetch and clear it before ladds_conjunctions");
-  ScanCounters sc{};
+  if (ctx->log_nonempty) return ctx->fail(LADDS_E_STATE, "pair log not empty: fetch and clear it before ladds_conjunctions");
+  // One synchronisation: the counters and the first entries of the log (pinned staging)
+  // come back together; more entries than staged, or than the log holds, cost a second.
   uint64_t entries = 0;
-  int rc = run_detection_sync(ctx, dt_interp, false, &sc, &entries);
-  if (rc != LADDS_OK) return rc;
-  if (entries > ctx->log_cap) {
-    rc = ensure_log(ctx, entries);
-    if (rc == LADDS_OK) rc = run_detection_sync(ctx, dt_interp, false, &sc, &entries);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
+    int rc = launch_detection(ctx, false, dt_interp, false, 0);
     if (rc != LADDS_OK) return rc;
+    if (!ctx->h_log) {
+      HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&ctx->h_log), sizeof(LoggedPair) * 4096, 0));
+      ctx->h_log_cap = 4096;
+    }
+    const size_t staged = std::min<size_t>(ctx->h_log_cap, ctx->log_cap);
+    HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->sc, ctx->counters, sizeof(ScanCounters), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->log_count, ctx->log_count, sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_log, ctx->log, sizeof(LoggedPair) * staged, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->h_rb->sc.overflow) return ctx->fail(LADDS_E_OVERFLOW, "conjunction scan scratch arena exhausted");
+    entries = ctx->h_rb->log_count;
+    if (entries > ctx->log_cap) {  // the log was too small: grow it and scan again
+      rc = ensure_log(ctx, entries);
+      if (rc != LADDS_OK) return rc;
+      continue;
+    }
+    if (entries > staged) {  // more than the staging holds: grow it, copy the rest
+      size_t want = ctx->h_log_cap;
+      while (want < entries) want *= 2;
+      LoggedPair *bigger = nullptr;
+      HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&bigger), sizeof(LoggedPair) * want, 0));
+      std::memcpy(bigger, ctx->h_log, sizeof(LoggedPair) * staged);
+      hipHostFree(ctx->h_log);
+      ctx->h_log = bigger;
+      ctx->h_log_cap = want;
+      HIPCHK(ctx, hipMemcpyAsync(ctx->h_log + staged, ctx->log + staged, sizeof(LoggedPair) * (entries - staged),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    break;
   }
-  std::vector<LoggedPair> h(entries);
-  if (entries)
-    HIPCHK(ctx, hipMemcpyAsync(h.data(), ctx->log, sizeof(LoggedPair) * entries, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   size_t nh = 0, nev = 0;
-  for (const LoggedPair &lp : h) {
+  for (uint64_t k = 0; k < entries; ++k) {
+    const LoggedPair &lp = ctx->h_log[k];
     if (lp.evaded) {
       if (nev < ecap) std::memcpy(&evaded[nev], &lp.pair, sizeof(ladds_pair));
       ++nev;
@@ -1314,6 +1418,40 @@ int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_
   *n = nh;
   *ne = nev;
   if (nh > cap || nev > ecap) return ctx->fail(LADDS_E_OVERFLOW, "pair output capacity too small");
+  return LADDS_OK;
+}
+
+int ladds_gather_state(ladds_ctx *ctx, const uint32_t *idx, size_t count, double *r, double *v) {
+  if (!ctx || (count && (!idx || !r || !v))) return LADDS_E_INVALID;
+  if (count == 0) return LADDS_OK;
+  for (size_t k = 0; k < count; ++k)
+    if (idx[k] >= ctx->n) return ctx->fail(LADDS_E_INVALID, "object index out of range");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t in_b = sizeof(uint32_t) * count, out_b = sizeof(double) * 6 * count;
+  const size_t bytes = (in_b + 255) / 256 * 256 + out_b;
+  int rc = ensure_xbuf(ctx, bytes);
+  if (rc != LADDS_OK) return rc;
+  if (bytes > ctx->h_x_bytes) {
+    if (ctx->h_x) hipHostFree(ctx->h_x);
+    ctx->h_x = nullptr;
+    ctx->h_x_bytes = 0;
+    HIPCHK(ctx, hipHostMalloc(&ctx->h_x, bytes, 0));
+    ctx->h_x_bytes = bytes;
+  }
+  char *hx = static_cast<char *>(ctx->h_x), *dx = static_cast<char *>(ctx->xbuf);
+  const size_t out_off = (in_b + 255) / 256 * 256;
+  std::memcpy(hx, idx, in_b);
+  HIPCHK(ctx, hipMemcpyAsync(dx, hx, in_b, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, launch_gather_state(ctx->d, reinterpret_cast<const uint32_t *>(dx), count,
+                                  reinterpret_cast<double *>(dx + out_off), ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(hx + out_off, dx + out_off, out_b, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const double *o = reinterpret_cast<const double *>(hx + out_off);
+  for (size_t k = 0; k < count; ++k)
+    for (int c = 0; c < 3; ++c) {
+      r[3 * k + c] = o[6 * k + c];
+      v[3 * k + c] = o[6 * k + 3 + c];
+    }
   return LADDS_OK;
 }
 
@@ -1361,6 +1499,16 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   const bool use_graph = ctx->cfg.step_graph != 0 && tpcd == 1 && ctx->variant == kVariantPlaneTiles &&
                          !ctx->profiling && ctx->n > 0 && nsteps > 0;
   const bool use_overlap = !use_graph && ctx->variant == kVariantPlaneTiles && ctx->n > 0 && nsteps > 0;
+  // Spawn protection on the device while objects with a parent id exist: every detection
+  // accounts the iterations since the previous one (span), trailing iterations at the end.
+  const bool spawn = ctx->protected_objects && ctx->n > 0 && nsteps > 0;
+  uint64_t next_acc = first_iteration;  // first iteration whose start is not accounted yet
+  if (spawn) {
+    if (!ctx->d_spawn) HIPCHK(ctx, dalloc(&ctx->d_spawn, 1));
+    ctx->h_rb->spawn = SpawnState{ctx->quiet, 0ull, 0ull, 0ull};
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_spawn, &ctx->h_rb->spawn, sizeof(SpawnState), hipMemcpyHostToDevice,
+                               ctx->stream));
+  }
   int cur = 0;                       // state buffer holding the latest state
   bool pending[2] = {false, false};  // a pair scan reading buffer b may still run on stream2
   if (use_overlap || use_graph) {
@@ -1385,99 +1533,112 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   }
   for (int seg0 = 0; seg0 < nsteps && ctx->n > 0; seg0 += kEnvSegment) {
     const int seg = std::min(kEnvSegment, nsteps - seg0);
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // pinned environment buffer is free again
+    // environments of the whole segment, in stream order after the previous segment's steps
+    {
+      const int rc = stage_env(ctx, 0, seg);
+      if (rc != LADDS_OK) {
+        drop_graphs();
+        return rc;
+      }
+    }
     if (use_graph) {
       // prologue: step 0 of the segment propagated and partitioned eagerly; steps j = 0..seg-2
       // replay graph(buffer of step j): pair scan of step j beside step j+1's propagation;
       // epilogue: the pair scan of the segment's last step.
       const uint64_t base = first_iteration + (uint64_t)seg0;
-      int staged = 0;
-      auto stage_to = [&](int need) {  // environments of steps [0, need) staged
-        while (staged < need) {
-          const int k = std::min(kChunk, seg - staged);
-          const int rc = stage_env(ctx, staged, k);
-          if (rc != LADDS_OK) return rc;
-          staged += k;
-        }
-        return (int)LADDS_OK;
-      };
-      int rc = stage_to(std::min(seg, 2));
-      hipError_t e = hipSuccess;
-      if (rc == LADDS_OK) {
-        e = launch_set_step(ctx->d_step, 0u, base, ctx->stream);
-        if (e == hipSuccess) rc = launch_prop_part(ctx, cur, 0, 1, true, burnup_r2, nullptr, 0);
-        cur = 1 - cur;
-      }
+      int rc = LADDS_OK;
+      hipError_t e = launch_set_step(ctx->d_step, 0u, base, ctx->stream);
+      if (e == hipSuccess) rc = launch_prop_part(ctx, cur, 0, 1, true, burnup_r2, nullptr, 0);
+      cur = 1 - cur;
       for (int j = 0; j + 1 < seg && rc == LADDS_OK && e == hipSuccess; ++j) {
-        rc = stage_to(std::min(seg, j + 2));
-        if (rc == LADDS_OK) e = hipGraphLaunch(graphs[cur], ctx->stream);
+        e = hipGraphLaunch(graphs[cur], ctx->stream);
         cur = 1 - cur;
       }
       if (rc == LADDS_OK && e == hipSuccess)
-        rc = launch_pairs_of(ctx, cur, ctx->cfg.dt, digest, base + (uint64_t)(seg - 1), nullptr, ctx->stream);
+        rc = launch_pairs_of(ctx, cur, ctx->cfg.dt, digest, base + (uint64_t)(seg - 1), nullptr, ctx->stream,
+                             spawn ? 1 : 0);
       if (rc == LADDS_OK && e != hipSuccess) rc = ctx->hipfail(e, "step graph launch");
       if (rc != LADDS_OK) {
         drop_graphs();
         return rc;
       }
+      next_acc = base + (uint64_t)seg;
       ctx->step_count += (uint64_t)seg;
       continue;
     }
     int s = 0;
-    int staged = 0;
     while (s < seg) {
-      if (s >= staged) {
-        const int k = std::min(kChunk, seg - staged);
-        const int rc = stage_env(ctx, staged, k);
-        if (rc != LADDS_OK) return rc;
-        staged += k;
-      }
       // integrate iterations up to (and including) the next detection iteration
       int k = 1;
       uint64_t it = first_iteration + (uint64_t)(seg0 + s);
-      while (s + k < staged && it % (uint64_t)tpcd != 0) {
+      while (s + k < seg && it % (uint64_t)tpcd != 0) {
         ++k;
         ++it;
       }
       const bool detect = (it % (uint64_t)tpcd) == 0;
+      const uint64_t span = (spawn && detect) ? it - next_acc + 1 : 0;
+      if (detect) next_acc = spawn ? it + 1 : next_acc;
       int rc;
       if (use_overlap) {
         rc = overlapped_iteration(ctx, cur, pending[1 - cur], (size_t)s, k, detect, burnup_r2,
-                                  ctx->cfg.dt * (double)tpcd, digest, it);
+                                  ctx->cfg.dt * (double)tpcd, digest, it, span);
         if (rc == LADDS_OK) {
           if (detect) pending[1 - cur] = true;
           cur = 1 - cur;
         }
       } else {
         rc = launch_propagate_steps(ctx, (size_t)s, k, detect, burnup_r2);
-        if (rc == LADDS_OK && detect) rc = launch_detection(ctx, true, ctx->cfg.dt * (double)tpcd, digest, it);
+        if (rc == LADDS_OK && detect)
+          rc = launch_detection(ctx, true, ctx->cfg.dt * (double)tpcd, digest, it, nullptr, span);
       }
       if (rc != LADDS_OK) return rc;
       s += k;
     }
     ctx->step_count += (uint64_t)seg;
   }
+  // join the pair-scan stream, account trailing iterations, read the counters back: one
+  // synchronisation
   if (use_overlap || use_graph) {
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream2));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    drop_graphs();
-    if (cur == 1) {  // the final state is in the second buffer: make it the context's own
-      std::swap(ctx->d.rx, ctx->sb_r[0]);
-      std::swap(ctx->d.ry, ctx->sb_r[1]);
-      std::swap(ctx->d.rz, ctx->sb_r[2]);
-      std::swap(ctx->d.vx, ctx->sb_v[0]);
-      std::swap(ctx->d.vy, ctx->sb_v[1]);
-      std::swap(ctx->d.vz, ctx->sb_v[2]);
-      std::swap(ctx->d.activity, ctx->sb_act);
-    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev_join, ctx->stream2));
+    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   }
-  ScanCounters sc{};
-  unsigned long long cnt = 0;
-  HIPCHK(ctx, hipMemcpyAsync(&sc, ctx->counters, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->log_count, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  const uint64_t end_it = first_iteration + (uint64_t)nsteps;
+  if (spawn && end_it > next_acc) {
+    HIPCHK(ctx, spawn_pre(ctx, end_it - next_acc, ctx->stream));
+    HIPCHK(ctx, spawn_post(ctx, end_it - next_acc, false, ctx->stream));
+  }
+  HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->sc, ctx->counters, sizeof(ScanCounters), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->log_count, ctx->log_count, sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  if (spawn)
+    HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->spawn, ctx->d_spawn, sizeof(SpawnState), hipMemcpyDeviceToHost,
+                               ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  drop_graphs();
+  if ((use_overlap || use_graph) && cur == 1) {  // the final state is in the second buffer: make it the context's own
+    std::swap(ctx->d.rx, ctx->sb_r[0]);
+    std::swap(ctx->d.ry, ctx->sb_r[1]);
+    std::swap(ctx->d.rz, ctx->sb_r[2]);
+    std::swap(ctx->d.vx, ctx->sb_v[0]);
+    std::swap(ctx->d.vy, ctx->sb_v[1]);
+    std::swap(ctx->d.vz, ctx->sb_v[2]);
+    std::swap(ctx->d.activity, ctx->sb_act);
+  }
+  const ScanCounters sc = ctx->h_rb->sc;
+  const unsigned long long cnt = ctx->h_rb->log_count;
+  ctx->log_nonempty = cnt != 0;
+  // spawn-protection count at the start of the next iteration (Simulation.cpp:399-402, 438)
+  if (spawn) {
+    ctx->quiet = ctx->h_rb->spawn.quiet;
+    if (ctx->h_rb->spawn.cleared) ctx->protected_objects = false;
+  } else if (sc.last_hit) {
+    ctx->quiet = end_it - (sc.last_hit - 1ull);
+  } else {
+    ctx->quiet += (uint64_t)nsteps;
+  }
   if (sc.overflow) return ctx->fail(LADDS_E_OVERFLOW, "conjunction scan scratch arena exhausted");
-  if (min_altitude_km > 0.0) {
+  // burnt-up objects were flagged by the propagation; compaction only when there are some
+  if (min_altitude_km > 0.0 && sc.burnups > 0) {
     const int rc = compact_if_needed(ctx);
     if (rc != LADDS_OK) return rc;
   }
@@ -1492,6 +1653,15 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     stats->edigest[1] = sc.edigest[1];
     stats->remaining = ctx->n;
   }
+  return LADDS_OK;
+}
+
+int ladds_remove_spawn_protection(ladds_ctx *ctx) {
+  if (!ctx) return LADDS_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, launch_clear_parents(nullptr, 1, ctx->d.parent, ctx->n, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->protected_objects = false;
   return LADDS_OK;
 }
 
@@ -1513,20 +1683,6 @@ int ladds_fetch_log(ladds_ctx *ctx, ladds_logged_pair *out, size_t cap, size_t *
 
 }  // extern "C"
 
-namespace {
-
-int ensure_xbuf(ladds_ctx *ctx, size_t bytes) {
-  if (!ctx->xcount) HIPCHK(ctx, dalloc(&ctx->xcount, 2));
-  if (bytes <= ctx->xbuf_bytes) return LADDS_OK;
-  hipFree(ctx->xbuf);
-  ctx->xbuf = nullptr;
-  ctx->xbuf_bytes = 0;
-  HIPCHK(ctx, hipMalloc(&ctx->xbuf, bytes));
-  ctx->xbuf_bytes = bytes;
-  return LADDS_OK;
-}
-
-}  // namespace
 
 extern "C" {
 
@@ -1581,6 +1737,7 @@ int ladds_clear_log(ladds_ctx *ctx) {
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->log_nonempty = false;
   return LADDS_OK;
 }
 

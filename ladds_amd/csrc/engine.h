@@ -116,11 +116,25 @@ struct LoggedPair {  // same layout as ladds_logged_pair
   uint32_t scenario;
 };
 
-// Counters written by the pair scan (device memory).
+// Counters written by the pair scan and the propagation (device memory).
 struct ScanCounters {
   unsigned long long hits, evaded, candidates;
   unsigned long long overflow;  // planes skipped because the scratch arena was exhausted
   unsigned long long digest[2], edigest[2];
+  unsigned long long burnups;   // objects the propagation flagged as burnt up
+  unsigned long long last_hit;  // 1 + the latest iteration that had a collision (0: none)
+};
+
+// Spawn protection (Simulation.cpp:287-293, 399-402, 438, 470-473): the simulation loop
+// counts iterations since the last one whose detection found collisions; when the count
+// is 100 at the start of an iteration, every parent id is reset, so fragments of one
+// breakup can conjunct again.  Device copy of the count, used while objects with a parent
+// id exist.
+struct SpawnState {
+  unsigned long long quiet;    // the count at the start of the first iteration not yet accounted
+  unsigned long long seen;     // ScanCounters::hits when the last detection was accounted
+  unsigned long long cleared;  // parent ids were reset
+  unsigned long long pad;
 };
 
 struct FunctorParams {

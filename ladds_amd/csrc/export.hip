@@ -73,6 +73,29 @@ __global__ __launch_bounds__(kThreads) void k_log_records(const LoggedPair *log,
   }
 }
 
+// Gather of the objects a conjunction list names (the drop-in adapter refreshes only the
+// Particle entries a detection returns, instead of downloading every object).
+__global__ __launch_bounds__(kThreads) void k_gather_state(DeviceObjects o, const uint32_t *idx, size_t count,
+                                                           double *out) {
+  const size_t k = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t i = idx[k];
+  double *d = out + 6 * k;
+  d[0] = o.rx[i];
+  d[1] = o.ry[i];
+  d[2] = o.rz[i];
+  d[3] = o.vx[i];
+  d[4] = o.vy[i];
+  d[5] = o.vz[i];
+}
+
+hipError_t launch_gather_state(const DeviceObjects &o, const uint32_t *idx, size_t count, double *out, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((count + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_gather_state, dim3(blocks), dim3(kThreads), 0, s, o, idx, count, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_particle_records(const DeviceObjects &o, size_t n, float *pos, float *vel, uint32_t *ids,
                                    hipStream_t s) {
   if (n == 0) return hipSuccess;

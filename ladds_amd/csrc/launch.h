@@ -13,7 +13,8 @@ hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<d
                             const SubstepEnv<float> *env32, int nsteps, unsigned mask, const double cdt[4],
                             const double ddt[3], const GridParams *grid, uint32_t *keys, uint32_t *count,
                             uint32_t *slot, double burnup_r2, const StepState *step_state, hipStream_t stream,
-                            const DeviceObjects *out = nullptr, uint32_t step_offset = 0);
+                            const DeviceObjects *out = nullptr, uint32_t step_offset = 0,
+                            unsigned long long *burnups = nullptr);
 // Graph replay bookkeeping: set the step position, advance it by one step.
 hipError_t launch_set_step(StepState *st, uint32_t step, uint64_t iter_base, hipStream_t stream);
 hipError_t launch_advance_step(StepState *st, hipStream_t stream);
@@ -61,8 +62,18 @@ hipError_t launch_compact(const DeviceObjects &src, const DeviceObjects &dst, co
 
 constexpr uint32_t kCompactTile = 2048;
 
+// Spawn protection.  launch_clear_parents with st == nullptr resets every parent id; with a
+// SpawnState it does so only when the quiet-iteration count reaches 100 within the next
+// `span` iteration starts.  launch_spawn_post then advances the count over those
+// iterations (resetting it when `detect` and the scan found collisions).
+hipError_t launch_clear_parents(SpawnState *st, uint64_t span, uint64_t *parent, size_t n, hipStream_t s);
+hipError_t launch_spawn_post(SpawnState *st, uint64_t span, const ScanCounters *counters, bool detect,
+                             hipStream_t s);
+
 // Output records (export.hip): float32 AoS state + uint32 ids; one iteration's pairs as
 // {uint32, uint32, float32}.  counts[2] must be zero before launch_log_records.
+// State of the objects at the given indices, AoS {rx, ry, rz, vx, vy, vz} per entry.
+hipError_t launch_gather_state(const DeviceObjects &o, const uint32_t *idx, size_t count, double *out, hipStream_t s);
 hipError_t launch_particle_records(const DeviceObjects &o, size_t n, float *pos, float *vel, uint32_t *ids,
                                    hipStream_t s);
 hipError_t launch_log_records(const LoggedPair *log, const unsigned long long *log_count, uint64_t log_cap,

@@ -115,6 +115,10 @@ struct PairSink {
   const StepState *step_state;  // non-null (graph replay): iteration = iter_base + step
 };
 
+__device__ __forceinline__ unsigned long long sink_iteration(const PairSink &a) {
+  return a.step_state ? a.step_state->iter_base + a.step_state->step : a.iteration;
+}
+
 struct ScanAcc {
   unsigned long long cand = 0, hit = 0, ev = 0, d0 = 0, d1 = 0, e0 = 0, e1 = 0;
 };
@@ -151,7 +155,10 @@ __device__ __forceinline__ void handle_candidate(const PairSink &a, uint32_t oi,
       unsigned long long base = 0;
       if ((int)lane_id() == leader) {
         base = atomicAdd(a.log_count, (unsigned long long)__popcll(all));
-        if (hm) atomicAdd(&a.counters->hits, (unsigned long long)__popcll(hm));
+        if (hm) {
+          atomicAdd(&a.counters->hits, (unsigned long long)__popcll(hm));
+          atomicMax(&a.counters->last_hit, sink_iteration(a) + 1ull);
+        }
         if (em) atomicAdd(&a.counters->evaded, (unsigned long long)__popcll(em));
       }
       base = __shfl(base, leader, 64);
@@ -161,7 +168,7 @@ __device__ __forceinline__ void handle_candidate(const PairSink &a, uint32_t oi,
         if (pos < a.log_cap) {
           LoggedPair lp;
           lp.pair = r;
-          lp.iteration = a.step_state ? a.step_state->iter_base + a.step_state->step : a.iteration;
+          lp.iteration = sink_iteration(a);
           lp.evaded = (kind == 2) ? 1u : 0u;
           lp.scenario = scenario;
           a.log[pos] = lp;
@@ -182,6 +189,7 @@ __device__ __forceinline__ void flush_counters(const PairSink &a, const ScanAcc 
     if (lane_id() == 0) {
       if (hit) {
         atomicAdd(&a.counters->hits, hit);
+        atomicMax(&a.counters->last_hit, sink_iteration(a) + 1ull);
         atomicAdd(&a.counters->digest[0], d0);
         atomicAdd(&a.counters->digest[1], d1);
       }

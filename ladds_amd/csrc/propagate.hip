@@ -35,6 +35,7 @@ struct PropagateArgs {
   uint32_t *count;  // non-null: counting pass of the atomic counting sort
   uint32_t *slot;
   double burnup_r2;  // > 0: objects with |r|^2 below become dummies (deleteBurnUps)
+  unsigned long long *burnups;  // non-null: count of objects flagged here
   const StepState *step_state;  // non-null (graph replay): environments start at step_state->step + step_offset
   uint32_t step_offset;
   // output state (the input arrays of `o` when propagating in place)
@@ -92,8 +93,10 @@ __global__ __launch_bounds__(256) void k_propagate(PropagateArgs a) {
       break;
     }
   }
-  if (burnt)
+  if (burnt) {
     a.oact[i] = -1;
+    if (a.burnups) atomicAdd(a.burnups, 1ull);  // rare: one atomic per burnt-up object
+  }
   else if (a.oact != a.o.activity)
     a.oact[i] = p.activity;
   a.orx[i] = x;
@@ -122,7 +125,7 @@ hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<d
                             const SubstepEnv<float> *env32, int nsteps, unsigned mask, const double cdt[4],
                             const double ddt[3], const GridParams *grid, uint32_t *keys, uint32_t *count,
                             uint32_t *slot, double burnup_r2, const StepState *step_state, hipStream_t stream,
-                            const DeviceObjects *out, uint32_t step_offset) {
+                            const DeviceObjects *out, uint32_t step_offset, unsigned long long *burnups) {
   if (o.n == 0 || nsteps <= 0) return hipSuccess;
   PropagateArgs a;
   a.o = o;
@@ -145,6 +148,7 @@ hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<d
   a.count = count;
   a.slot = slot;
   a.burnup_r2 = burnup_r2;
+  a.burnups = burnups;
   a.step_state = step_state;
   a.step_offset = step_offset;
   const unsigned blocks = (unsigned)((o.n + 255) / 256);

@@ -11,12 +11,20 @@
 //   particle <rx ry rz vx vy vz id activity mass radius_m bc_inv parent cr>
 //   integrate <steps>
 //   functor <cutoff dt cdf min_det_radius_m evasion_cutoff_km>   -> "hit|evaded idA idB d2 mx my mz"
+//                           and "hitpos id rx ry rz vx vy vz" of both Particle* of every tuple
 //   state                                                         -> "state id rx ry rz vx vy vz"
 //   burnups <min_altitude_km>                                     -> "removed n"
 //   simulate <first_iteration> <steps> <tpcd> <min_altitude_km>   -> "stats ..." and "log ..."
 //   conjlog <path>          ConjunctionLogger CSV of the last simulate's log
+//   unprotect               Simulation::removeParticleSpawnProtection
+//   population <workload> <seed> <n> <planted_pairs>   seeded synthetic objects (ladds_generate_population)
+//   timeloop <steps> <cutoff dt cdf min_det_radius_m evasion_cutoff_km>
+//                           the per-iteration drop-in sequence of Simulation.cpp:297 and :211-222,
+//                           integrate(false) + a fresh CollisionFunctor + iteratePairwise, timed
+//                           -> "timing steps seconds collisions evasions"
 //   dataset <path> <c_D>    DatasetReader::readDataset (no device needed)
 //                           -> "data id cospar activity mass radius bc_inv rx ry rz vx vy vz"
+#include <chrono>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -31,6 +39,16 @@ using namespace ladds_amd;
 static void print_pairs(const char *tag, const CollisionFunctor::CollisionCollectionT &c) {
   for (const auto &[a, b, d2, mid] : c)
     std::printf("%s %zu %zu %.17g %.17g %.17g %.17g\n", tag, a->getID(), b->getID(), d2, mid[0], mid[1], mid[2]);
+}
+
+// What the tuples' Particle* hold (the adapter refreshes only these objects per detection).
+static void print_hit_particles(const CollisionFunctor::CollisionCollectionT &c) {
+  for (const auto &t : c)
+    for (const Particle *p : {std::get<0>(t), std::get<1>(t)}) {
+      const auto &r = p->getPosition();
+      const auto &v = p->getVelocity();
+      std::printf("hitpos %zu %.17g %.17g %.17g %.17g %.17g %.17g\n", p->getID(), r[0], r[1], r[2], v[0], v[1], v[2]);
+    }
 }
 
 // Every field of the line was read and nothing but blanks follows.
@@ -125,6 +143,43 @@ int main(int argc, char **argv) {
         container->iteratePairwise(&f);
         print_pairs("hit", f.getCollisions());
         print_pairs("evaded", f.getEvadedCollisions());
+        print_hit_particles(f.getCollisions());
+        print_hit_particles(f.getEvadedCollisions());
+      } else if (cmd == "unprotect") {
+        if (!parsed(ls, line)) return 2;
+        container->removeParticleSpawnProtection();
+      } else if (cmd == "population") {
+        int workload = 0;
+        unsigned long long seed = 0, n = 0, planted = 0;
+        ls >> workload >> seed >> n >> planted;
+        if (!parsed(ls, line)) return 2;
+        std::vector<double> a(11 * n);
+        double *c[11];
+        for (int k = 0; k < 11; ++k) c[k] = a.data() + k * n;
+        check(ladds_generate_population(workload, seed, n, planted, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7],
+                                        c[8], c[9], c[10]),
+              nullptr, "ladds_generate_population");
+        for (size_t i = 0; i < n; ++i)
+          container->addParticle(Particle({c[0][i], c[1][i], c[2][i]}, {c[3][i], c[4][i], c[5][i]}, i, "",
+                                          Particle::ActivityState::passive, c[7][i], c[6][i], c[9][i],
+                                          std::numeric_limits<size_t>::max(), c[10][i]));
+      } else if (cmd == "timeloop") {
+        int steps = 0;
+        double cutoff, dt, cdf, min_det, evasion;
+        ls >> steps >> cutoff >> dt >> cdf >> min_det >> evasion;
+        if (!parsed(ls, line)) return 2;
+        container->syncDevice();  // the population upload is not part of the loop
+        unsigned long long hits = 0, evasions = 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int s = 0; s < steps; ++s) {
+          integrator->integrate(false);
+          CollisionFunctor f(cutoff, dt, cdf, min_det, evasion);
+          container->iteratePairwise(&f);
+          hits += f.getCollisions().size();
+          evasions += f.getEvadedCollisions().size();
+        }
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("timing %d %.9g %llu %llu\n", steps, sec, hits, evasions);
       } else if (cmd == "state") {
         for (auto it = container->cbegin(); it != container->cend(); ++it) {
           const auto &r = it->getPosition();

@@ -28,13 +28,17 @@ def _run(tmp_path, lines):
     scen.write_text("\n".join(lines) + "\n")
     res = subprocess.run([str(EXE), str(scen)], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
-    out = {"hit": [], "evaded": [], "state": {}, "removed": [], "stats": [], "log": []}
+    out = {"hit": [], "evaded": [], "state": {}, "removed": [], "stats": [], "log": [], "hitpos": {}, "timing": []}
     for ln in res.stdout.splitlines():
         tag, *f = ln.split()
         if tag in ("hit", "evaded"):
             out[tag].append((int(f[0]), int(f[1]), float(f[2]), [float(x) for x in f[3:6]]))
         elif tag == "state":
             out["state"][int(f[0])] = np.array([float(x) for x in f[1:7]])
+        elif tag == "hitpos":
+            out["hitpos"][int(f[0])] = np.array([float(x) for x in f[1:7]])
+        elif tag == "timing":
+            out["timing"].append((int(f[0]), float(f[1]), int(f[2]), int(f[3])))
         elif tag == "removed":
             out["removed"].append(int(f[0]))
         elif tag == "stats":
@@ -241,3 +245,35 @@ def test_breakup_and_conjunction_log_match_python_binding(tmp_path):
         return sorted((int(a), int(b), int(c), float(d)) for a, b, c, d in (ln.split(",") for ln in text[1:]))
 
     assert rows(tmp_path / "cpp.csv") == rows(tmp_path / "py.csv") and len(rows(tmp_path / "py.csv")) > 0
+
+
+def test_iterate_pairwise_refreshes_the_particles_it_returns(tmp_path):
+    """After integrate(false), iteratePairwise brings only the objects its tuples point at up
+    to date (no download of the population): their Particle positions and velocities equal
+    the device state that a full read of the container returns afterwards."""
+    pop = la.generate_population(0, 0x1ADD5 + 10, 3000, planted_pairs=40)
+    lines = _header(10.0, 10.0, comps=(1, 1, 0, 0, 0, 0, 0, 0))
+    for i in range(pop.n):
+        lines.append(_particle((pop.rx[i], pop.ry[i], pop.rz[i]), (pop.vx[i], pop.vy[i], pop.vz[i]), i,
+                               mass=pop.mass_kg[i], radius=pop.radius_m[i], bc=pop.bc_inv[i], cr=pop.cr[i]))
+    lines += ["integrate 3", "functor 10.0 10.0 200000.0 0.05 0.1", "state"]
+    out = _run(tmp_path, lines)
+    assert len(out["hit"]) >= 20 and len(out["hitpos"]) >= 20
+    for pid, st in out["hitpos"].items():
+        assert np.array_equal(st, out["state"][pid]), pid
+    moved = [pid for pid in out["hitpos"] if not np.array_equal(out["state"][pid][:3], pop.r()[pid])]
+    assert len(moved) == len(out["hitpos"])
+
+
+def test_remove_particle_spawn_protection(tmp_path):
+    """Simulation::removeParticleSpawnProtection through the adapter: fragments of one parent
+    are skipped by the functor until their parent ids are reset."""
+    rng = np.random.default_rng(4)
+    lines = _header(10.0, 10.0)
+    centre = np.array([7000.0, 100.0, 50.0])
+    for k in range(5):
+        lines.append(_particle(centre + rng.uniform(-0.2, 0.2, 3), [0.0, 7.5, 0.01 * k], 100 + k, radius=500.0,
+                               parent=42))
+    lines += ["functor 10.0 10.0 1.0 0.05 0.1", "unprotect", "functor 10.0 10.0 1.0 0.05 0.1"]
+    out = _run(tmp_path, lines)
+    assert len(out["hit"]) == 10  # none before the reset, all 10 pairs after it

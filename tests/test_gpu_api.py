@@ -1,0 +1,92 @@
+"""Context bookkeeping through the C ABI: the clock across configure calls, the shared
+pair log, the gather of object states, and the burn-up compaction of ladds_simulate."""
+import numpy as np
+import pytest
+
+import ladds_amd as la
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(**kw):
+    cfg = dict(dt=10.0, components=la.KEP | la.J2, cutoff_km=10.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
+               threshold_km=1.0)
+    cfg.update(kw)
+    return la.Engine(la.EngineConfig(**cfg))
+
+
+def test_clock_survives_configure_calls():
+    pop = la.generate_population(1, 17, 512)
+    with _engine(components=la.ALL_COMPONENTS) as eng:
+        eng.set_objects(pop)
+        eng.propagate(3)
+        assert eng.time == 30.0
+        eng.configure(components=la.KEP)  # components only: the clock keeps running
+        assert eng.time == 30.0
+        eng.configure(dt=5.0)  # a new step keeps the elapsed time
+        assert eng.time == 30.0
+        eng.propagate(2)
+        assert eng.time == 40.0
+        eng.set_time(1.0e6)
+        eng.configure(components=la.ALL_COMPONENTS)  # set_time then configure: the new clock stays
+        assert eng.time == 1.0e6
+        eng.propagate(1)
+        assert eng.time == 1.0e6 + 5.0
+        r, _ = eng.state()
+    # the Sun/Moon/Earth-rotation environment of the last step was evaluated at t = 1e6 s
+    r0, v0 = pop.r(), pop.v()
+    ro, vo = ol.propagate(r0, v0, 0.0, 10.0, 0xFF, 3, pop.aom, pop.bc_inv, pop.cr, pop.activity)
+    ro, vo = ol.propagate(ro, vo, 30.0, 5.0, la.KEP, 2)
+    ro, vo = ol.propagate(ro, vo, 1.0e6, 5.0, 0xFF, 1, pop.aom, pop.bc_inv, pop.cr, pop.activity)
+    assert np.max(np.linalg.norm(r - ro, axis=1) / np.linalg.norm(ro, axis=1)) < 1e-9
+
+
+def test_autotune_refuses_an_unfetched_log():
+    pop = la.generate_population(0, 0x1ADD5, 8192, planted_pairs=64)
+    with _engine() as eng:
+        eng.set_objects(pop)
+        st = eng.simulate(2, tpcd=1)
+        assert st["collisions"] > 0
+        with pytest.raises(la.LaddsError):
+            eng.autotune(repetitions=1)
+        with pytest.raises(la.LaddsError):
+            eng.conjunctions()
+        log = eng.fetch_log()
+        assert log.shape[0] == st["collisions"] + st["evasions"]
+        eng.clear_log()
+        eng.autotune(repetitions=1)
+        hits, _ = eng.conjunctions()
+        assert hits.shape[0] > 0
+
+
+def test_gather_state_equals_full_state():
+    pop = la.generate_population(1, 23, 20000)
+    with _engine(components=la.ALL_COMPONENTS) as eng:
+        eng.set_objects(pop)
+        eng.propagate(4)
+        idx = np.array([0, 19999, 7, 7, 12345, 500], np.uint32)
+        r, v = eng.gather_state(idx)
+        rf, vf = eng.state()
+        assert np.array_equal(r, rf[idx]) and np.array_equal(v, vf[idx])
+        with pytest.raises(la.LaddsError):
+            eng.gather_state([20000])
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_simulate_compacts_only_after_burnups(graph):
+    """Objects pushed below 150 km burn up inside ladds_simulate and are compacted at its
+    end, order kept; a call without burn-ups leaves the population as it is."""
+    pop = la.generate_population(0, 31, 5000)
+    r = pop.r()
+    low = np.arange(3, 5000, 97)
+    for k in low:
+        r[k] *= (6378.1363 + 140.0) / np.linalg.norm(r[k])
+    pop.rx, pop.ry, pop.rz = r[:, 0].copy(), r[:, 1].copy(), r[:, 2].copy()
+    with _engine(step_graph=graph) as eng:
+        eng.set_objects(pop)
+        st = eng.simulate(3, tpcd=1, min_altitude_km=150.0)
+        assert st["remaining"] == 5000 - low.size == eng.n
+        assert eng.ids().tolist() == np.setdiff1d(np.arange(5000), low).tolist()
+        st = eng.simulate(3, first_iteration=3, tpcd=1, min_altitude_km=150.0)
+        assert st["remaining"] == 5000 - low.size == eng.n
