@@ -105,14 +105,15 @@ __global__ __launch_bounds__(256) void k_bin_keys(const double *rx, const double
   keys[i] = cell_of(x, y, z, g).bucket;
 }
 
-// STAGE: records are placed in LDS first and the tile's region is then written with
-// contiguous stores (scattered 16/32-byte stores into the region cost ~3x the bytes in
-// partial-line write-backs).  Needs tile_size records of LDS.
-template <typename REC, int kPartItems, bool STAGE>
+// Records are placed in LDS first and the tile's region is then written with contiguous
+// stores (scattered 16/32-byte stores into the region cost ~3x the bytes in partial-line
+// write-backs).  A tile whose records exceed the LDS staging capacity (`stage_cap`
+// records) is staged in several passes over consecutive ranges of its region.
+template <typename REC, int kPartItems>
 __global__ __launch_bounds__(kPartThreads) void k_tile_partition(const double *rx, const double *ry,
                                                                   const double *rz, size_t n, const uint32_t *keys,
                                                                   TileParams tp, REC *items, uint32_t *tps,
-                                                                  uint32_t *arena_top) {
+                                                                  uint32_t *arena_top, uint32_t stage_cap) {
   extern __shared__ uint32_t lds[];
   uint32_t *cnt = lds;                // nplanes
   uint32_t *wsum = lds + tp.nplanes;  // kPartThreads / 64 + 1
@@ -157,27 +158,28 @@ __global__ __launch_bounds__(kPartThreads) void k_tile_partition(const double *r
       pz[k] = rz[i];
     }
   }
+  uint32_t pos[kPartItems];
 #pragma unroll
-  for (int k = 0; k < kPartItems; ++k) {
-    if (key[k] == ~0u) continue;
-    const size_t i = base + threadIdx.x + (size_t)k * kPartThreads;
-    const uint32_t pos = atomicAdd(&cnt[key[k] >> pb], 1u);
-    double x = px[k], y = py[k], z = pz[k];
-    binning_point<f32>(x, y, z);
-    REC r;
-    r.x = x;
-    r.y = y;
-    r.z = z;
-    r.idx = (uint32_t)i;
-    if constexpr (!f32) r.pad = 0;
-    if constexpr (STAGE)
-      staged[pos] = r;
-    else
-      items[base + pos] = r;
-  }
-  if constexpr (STAGE) {
+  for (int k = 0; k < kPartItems; ++k) pos[k] = key[k] != ~0u ? atomicAdd(&cnt[key[k] >> pb], 1u) : ~0u;
+#pragma unroll 1
+  for (uint32_t lo = 0; lo < total; lo += stage_cap) {
+    const uint32_t hi = min(total, lo + stage_cap);
+#pragma unroll
+    for (int k = 0; k < kPartItems; ++k) {
+      if (pos[k] < lo || pos[k] >= hi) continue;
+      double x = px[k], y = py[k], z = pz[k];
+      binning_point<f32>(x, y, z);
+      REC r;
+      r.x = x;
+      r.y = y;
+      r.z = z;
+      r.idx = (uint32_t)(base + threadIdx.x + (size_t)k * kPartThreads);
+      if constexpr (!f32) r.pad = 0;
+      staged[pos[k] - lo] = r;
+    }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < total; k += kPartThreads) items[base + k] = staged[k];
+    for (uint32_t j = lo + threadIdx.x; j < hi; j += kPartThreads) items[base + j] = staged[j - lo];
+    __syncthreads();
   }
 }
 
@@ -244,27 +246,41 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
     const uint32_t zw = fz ? 0u : mz;  // the wrapped bucket when `wraps`
 #pragma unroll 1
     for (uint32_t kind = 0; kind < 6u; ++kind) {
-      const uint32_t *cur = (kind & 1u) ? cur_halo : cur_own;
-      const uint32_t row = ((fy + (kind >> 1) - 1u) & my) << sy;
-      const uint32_t b_lo = row | zlo, b_w = row | zw;
-      const uint32_t lo0 = cur[(int32_t)b_lo - 1], hi0 = cur[row | zhi];
-      const uint32_t w0 = cur[(int32_t)b_w - 1], w1 = cur[b_w];
-      const uint32_t lo1 = wraps ? w0 : 0u, hi1 = wraps ? w1 : 0u;
       const bool next_plane = (kind & 1u) != 0u;
       // Own-plane pairs: with one workgroup per plane each pair is taken from its lower
       // sorted position.  The in-LDS placement order within a bucket depends on atomic
       // timing, so chunks of one plane (each sorting the plane itself) cannot share that
       // rule: they take a pair from its lower object index instead.
       const bool by_index = CHUNKED && !next_plane;
+      // own row y-1 sorts entirely below this record unless the fold wraps it (fy = 0)
+      if (!CHUNKED && kind == 0u && fy != 0u) continue;
+      const uint32_t *cur = next_plane ? cur_halo : cur_own;
+      const uint32_t row = ((fy + (kind >> 1) - 1u) & my) << sy;
+      const uint32_t b_lo = row | zlo, b_w = row | zw;
+      const uint32_t lo0 = cur[(int32_t)b_lo - 1], hi0 = cur[row | zhi];
+      const uint32_t w0 = cur[(int32_t)b_w - 1], w1 = cur[b_w];
+      const uint32_t lo1 = wraps ? w0 : 0u, hi1 = wraps ? w1 : 0u;
       const uint32_t from = (next_plane || by_index) ? 0u : s + 1u;
       uint32_t q = max(lo0, from), end = hi0;
 #pragma unroll 1
       for (int part = 0; part < 2; ++part) {
+        // two candidates per iteration: their LDS reads are issued together
 #pragma unroll 1
-        for (; q < end; ++q) {
-          const RecOf<FORM> other = sorted[q];
-          if (CHUNKED && by_index && other.idx <= me.idx) continue;
-          if (within<FORM>(me, other, a.sink.fp)) handle_candidate<DIGEST>(a.sink, me.idx, other.idx, acc, scen);
+        for (; q < end; q += 2u) {
+          const bool two = q + 1u < end;
+          const RecOf<FORM> o0 = sorted[q];
+          const RecOf<FORM> o1 = sorted[two ? q + 1u : q];
+          bool c0 = within<FORM>(me, o0, a.sink.fp);
+          bool c1 = two && within<FORM>(me, o1, a.sink.fp);
+          if (CHUNKED && by_index) {
+            c0 = c0 && o0.idx > me.idx;
+            c1 = c1 && o1.idx > me.idx;
+          }
+          if (c0 || c1) {
+#pragma unroll 1
+            for (int u = 0; u < 2; ++u)
+              if (u ? c1 : c0) handle_candidate<DIGEST>(a.sink, me.idx, u ? o1.idx : o0.idx, acc, scen);
+          }
         }
         q = max(lo1, from);
         end = hi1;
@@ -471,13 +487,12 @@ void launch_partition_items(const DeviceObjects &o, const uint32_t *keys, const 
                             uint32_t *tps, uint32_t *arena_top, hipStream_t s) {
   const dim3 grid(tp.ntiles), block(kPartThreads);
   const size_t head = sizeof(uint32_t) * partition_header_words(tp.nplanes);
-  const size_t staged = head + sizeof(REC) * (size_t)ITEMS * kPartThreads;
-  if (staged <= kPairTileLdsBytes)
-    hipLaunchKernelGGL((k_tile_partition<REC, ITEMS, true>), grid, block, staged, s, o.rx, o.ry, o.rz, o.n, keys, tp,
-                       out, tps, arena_top);
-  else
-    hipLaunchKernelGGL((k_tile_partition<REC, ITEMS, false>), grid, block, head, s, o.rx, o.ry, o.rz, o.n, keys, tp,
-                       out, tps, arena_top);
+  // staging capacity: the whole tile when it fits the LDS, else the largest multiple of
+  // 1024 records that does (two or more passes)
+  size_t cap = (size_t)ITEMS * kPartThreads;
+  if (head + sizeof(REC) * cap > kPairTileLdsBytes) cap = (kPairTileLdsBytes - head) / sizeof(REC) / 1024 * 1024;
+  hipLaunchKernelGGL((k_tile_partition<REC, ITEMS>), grid, block, head + sizeof(REC) * cap, s, o.rx, o.ry, o.rz, o.n,
+                     keys, tp, out, tps, arena_top, (uint32_t)cap);
 }
 
 template <typename REC>
@@ -529,15 +544,12 @@ hipError_t configure_tile_kernels() {
   auto set = [&](const void *f) {
     if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
   };
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 4, true>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 8, false>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 16, false>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 4, false>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 4, true>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 8, true>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 4, false>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 8, false>));
-  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 16, false>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 4>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 8>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF64, 16>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 4>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 8>));
+  set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 16>));
   return e;
 }
 
