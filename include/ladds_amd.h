@@ -166,6 +166,8 @@ LADDS_API void ladds_config_default(ladds_config *cfg);
  *      built by Simulation::initAutoPas (Simulation.cpp:59-120) ---- */
 LADDS_API int ladds_create(const ladds_config *cfg, ladds_ctx **out);
 LADDS_API int ladds_destroy(ladds_ctx *ctx);
+/* Message of the context's last failure; with NULL, that of the last failed
+ * ladds_conjunction_digest_noalloc call on this thread. */
 LADDS_API const char *ladds_last_error(const ladds_ctx *ctx);
 /* Update the parameters the integrator, accumulator and CollisionFunctor own (dt, t0,
  * components, functor thresholds, pair output, log capacity) -- the reference sets them in
@@ -327,6 +329,31 @@ LADDS_API int ladds_adjoint_forward(ladds_adjoint *plan, const double *r0, const
 LADDS_API int ladds_adjoint_backward(ladds_adjoint *plan, const double *aom, const double *bc_inv, const double *cr,
                                      const int32_t *activity, const double *checkpoints, const double *g_rT,
                                      const double *g_vT, double *g_r0, double *g_v0, double *g_bc, void *stream);
+
+/* ---- conjunction digest without allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the count
+ * and digest of ladds_conjunction_digest for objects in the caller's device memory, with
+ * the scan's scratch (cell keys, sorted records, tile offsets, arena, counters) in one
+ * static device array of each GPU, sized for at most LADDS_NOALLOC_MAX_OBJECTS objects
+ * (about 6.5 MB, held until the process ends).  Calls no hipMalloc / hipFree.  Uses
+ * cfg->cutoff_km, precision, threshold_mode, threshold_km, collision_distance_factor,
+ * min_detection_radius_m, evasion_cutoff_km and device; runs on `stream` (a hipStream_t,
+ * NULL: the default stream) and waits for it.  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one device race.  More objects than the maximum:
+ * LADDS_E_INVALID, message from ladds_last_error(NULL). */
+#define LADDS_NOALLOC_MAX_OBJECTS 65536
+typedef struct ladds_device_objects {
+  size_t n;
+  const double *rx, *ry, *rz;   /* device pointers, as ladds_objects       */
+  const double *vx, *vy, *vz;
+  const double *radius_m;
+  const int32_t *activity;
+  const uint64_t *id, *parent;
+} ladds_device_objects;
+LADDS_API int ladds_conjunction_digest_noalloc(const ladds_device_objects *objs, const ladds_config *cfg,
+                                               double dt_interp, uint64_t *n, uint64_t digest[2], uint64_t *ne,
+                                               uint64_t edigest[2], void *stream);
 
 /* ---- seeded synthetic populations (host only, no device needed) ----
  * workload 0: Keplerian elements as the 16,384-object case (perigee altitude U[200,2000] km,

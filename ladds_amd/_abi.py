@@ -138,6 +138,15 @@ try:
 except ImportError:  # pragma: no cover
     np = None
 
+class DeviceObjects(C.Structure):
+    """ladds_device_objects: device pointers of caller-owned object arrays."""
+    _fields_ = [("n", C.c_size_t), ("rx", C.c_void_p), ("ry", C.c_void_p), ("rz", C.c_void_p),
+                ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p), ("radius_m", C.c_void_p),
+                ("activity", C.c_void_p), ("id", C.c_void_p), ("parent", C.c_void_p)]
+
+
+NOALLOC_MAX_OBJECTS = 65536  # LADDS_NOALLOC_MAX_OBJECTS
+
 _ctx = C.c_void_p
 _SIGNATURES = {
     "ladds_version": (C.c_char_p, []),
@@ -186,6 +195,9 @@ _SIGNATURES = {
     "ladds_adjoint_forward": (C.c_int, [_ctx] + [C.c_void_p] * 10),
     "ladds_adjoint_backward": (C.c_int, [_ctx] + [C.c_void_p] * 11),
     "ladds_generate_population": (C.c_int, [C.c_int, C.c_uint64, C.c_size_t, C.c_size_t] + [_dp] * 11),
+    "ladds_conjunction_digest_noalloc": (C.c_int, [C.POINTER(DeviceObjects), C.POINTER(Config), C.c_double,
+                                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_void_p]),
 }
 
 

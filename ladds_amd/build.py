@@ -35,6 +35,7 @@ SOURCES = [
     "compact.hip",
     "tiles.hip",
     "adjoint.hip",
+    "noalloc.cpp",
 ]
 
 COMMON = [

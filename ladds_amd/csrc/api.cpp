@@ -25,6 +25,10 @@
 
 using namespace ladds;
 
+namespace ladds {
+const char *noalloc_error();
+}
+
 namespace {
 
 struct EventPair {
@@ -896,7 +900,11 @@ int ladds_destroy(ladds_ctx *ctx) {
   return LADDS_OK;
 }
 
-const char *ladds_last_error(const ladds_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *ladds_last_error(const ladds_ctx *ctx) {
+  if (ctx) return ctx->err.c_str();
+  const char *e = noalloc_error();  // the context-free entry point's last error on this thread
+  return e[0] ? e : "null context";
+}
 
 int ladds_configure(ladds_ctx *ctx, const ladds_config *cfg) {
   if (!ctx || !cfg) return LADDS_E_INVALID;
