@@ -100,6 +100,9 @@ def parse():
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default plane-tile shape (profiler runs: the tuner's timings change under "
                         "counter collection)")
+    p.add_argument("--tuning", metavar="JSON",
+                   help="apply these plane-tile shapes instead of autotuning: a JSON object per form, as the "
+                        "'tuning' field of an earlier run's line (profiler runs then measure the tuned shape)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="first copy MIB MiB device-to-device 3 times (a plain copy kernel of known size, "
                         "for scaling hardware byte counters), then run the bench")
@@ -123,11 +126,16 @@ def make_engine(la, precision):
     return la.Engine(cfg)
 
 
-def run_form(la, pop, precision, steps, warmup, profile_steps, autotune=True):
+def run_form(la, pop, precision, steps, warmup, profile_steps, autotune=True, pinned=None):
     eng = make_engine(la, precision)
     eng.set_objects(pop)
-    # plane-tile shape picked by timing the scan on the initial state (outside the timed region)
-    tuning = eng.autotune(repetitions=3) if autotune else None
+    # plane-tile shape picked by timing the scan on the initial state (outside the timed region),
+    # or a shape an earlier run picked
+    if pinned:
+        eng.set_tuning(pinned)
+        tuning = dict(pinned)
+    else:
+        tuning = eng.autotune(repetitions=3) if autotune else None
     if warmup:
         eng.simulate(warmup, first_iteration=0, tpcd=1, min_altitude_km=MIN_ALT)
     eng.clear_log()
@@ -314,7 +322,26 @@ def ensemble_leg(la, steps, warmup=10):
     return out
 
 
-def clustered_leg(la, steps):
+def clustered_cpu_baseline(la, pop, ids, r, v, gpu_digest):
+    """CPU leg of the clustered workload: the oracle's digest-only grid scan (OpenMP on the
+    host threads) of the state the GPU reached, timed, and whether its count and digest
+    equal the GPU's for that state (a full-size parity check of ~2.7e9 conjunctions)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as ol
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    k = ids.astype(np.int64)  # ids are the generated indices; burnt-up objects are gone
+    objs = ol.ObjectSet(r, v, pop.radius_m[k], pop.activity[k], pop.id[k], pop.parent[k])
+    t0 = time.perf_counter()
+    want = ol.grid_digest(objs, ol.functor_cfg(5.0, 10.0, threshold_mode=1, threshold_km=0.5), threads)
+    el = time.perf_counter() - t0
+    return {"kind": "port", "method": "oracle digest-only linked-cell scan of the GPU's final state",
+            "objects": int(pop.n), "seconds": el, "cores": threads,
+            "conjunctions_per_s": want[0] / el, "conjunctions": int(want[0]),
+            "digest_equals_gpu": bool(want == gpu_digest)}
+
+
+def clustered_leg(la, steps, cpu_check=True):
     """8,388,608 objects, 60 % replicated constellation slots (+-50 m) and 40 % background;
     split precision, dt = 10 s, cutoff 5 km, 0.5 km threshold, pair output as count + digest;
     the autotuner picks cell size and tile shape.  The FP64 and split forms' digests of the
@@ -340,6 +367,10 @@ def clustered_leg(la, steps):
                 el = time.perf_counter() - t0
                 out.update({"object_steps_per_s": n * steps / el, "ms_per_step": 1e3 * el / steps,
                             "conjunctions_per_step": st["collisions"] / max(steps, 1), "tuning": tuning})
+                if cpu_check:
+                    final = eng.conjunction_digest(10.0)
+                    r, v = eng.state()
+                    out["cpu_baseline"] = clustered_cpu_baseline(la, pop, eng.ids(), r, v, final)
     out["initial_conjunctions"] = int(digests["split"][0])
     out["split_digest_equals_fp64"] = bool(digests["split"] == digests["fp64"])
     return out
@@ -401,8 +432,10 @@ def main():
     print(f"# population {a.objects} generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
     forms = {"fp64": la.FP64, "split": la.SPLIT}
     res = {}
+    pinned = json.loads(Path(a.tuning).read_text() if a.tuning and Path(a.tuning).exists() else a.tuning or "{}")
     for name in a.forms.split(","):
-        res[name] = run_form(la, pop, forms[name], a.steps, a.warmup, a.profile_steps, not a.no_autotune)
+        res[name] = run_form(la, pop, forms[name], a.steps, a.warmup, a.profile_steps, not a.no_autotune,
+                             pinned.get(name))
         r = res[name]
         print(f"# {name}: {a.steps} steps in {r['elapsed']:.3f} s, {a.objects * a.steps / r['elapsed']:.4g} "
               f"object-steps/s, collisions {r['stats']['collisions']}, candidates {r['stats']['candidates']}, "
@@ -479,7 +512,7 @@ def main():
         print(f"# ensemble: {e['object_steps_per_s']:.4g} object-steps/s (eager {e['eager_object_steps_per_s']:.4g}),"
               f" graph==eager {e['graph_equals_eager']}, collisions {e['collisions']}", file=sys.stderr, flush=True)
     if not a.no_clustered:
-        out["clustered"] = clustered_leg(la, a.clustered_steps)
+        out["clustered"] = clustered_leg(la, a.clustered_steps, not a.no_cpu_baseline)
         print(f"# clustered: {json.dumps(out['clustered'])}", file=sys.stderr, flush=True)
     if not a.no_dropin:
         out["dropin"] = dropin_leg(a.dropin_steps)

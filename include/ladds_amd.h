@@ -287,6 +287,12 @@ typedef struct ladds_tuning {
 } ladds_tuning;
 
 LADDS_API int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuning *out);
+/* Apply a shape that ladds_autotune chose earlier (cell_size_km, plane_width_km,
+ * plane_bits, pair_threads, tile_size, chunks, workgroups_per_cu; the timing fields are
+ * ignored) without timing anything -- the counterpart of running LADDS with the
+ * calibrated AutoPas configuration that tuning mode writes out (Simulation.cpp:404-407,
+ * dumpCalibratedConfig).  A value outside the tuner's range is LADDS_E_INVALID. */
+LADDS_API int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *tuning);
 
 /* ---- measurement ---- */
 /* Record HIP events around every kernel launch; ladds_kernel_stats reports per kernel kind. */

@@ -113,6 +113,8 @@ struct ContainerConfig {
   int precision = LADDS_FP64;    /* LADDS_FP64 | LADDS_SPLIT | LADDS_SPLIT_ALL32          */
   int scanVariant = 0;           /* 0: automatic, 1: atomic counting sort, 2: plane tiles */
   int device = 0;                /* HIP device ordinal                                   */
+  bool tune = true;              /* tune the scan on the first iteratePairwise (AutoPas's
+                                    tuning phase) when the container holds enough objects */
 };
 
 /* The autopas::AutoPas<Particle> subset the hot path uses: particle storage, iteration,
@@ -174,8 +176,11 @@ class Container {
   const_iterator cend() const { return particles_.cend(); }
 
   /* AutoPas::iteratePairwise with a CollisionFunctor (Simulation.cpp:211-222).  Returns
-   * false: there is no tuning phase to report. */
+   * whether this call was a tuning iteration (AutoPas's return value): the first call on a
+   * container of at least kTuneMinObjects objects times the scan's shapes first
+   * (ladds_autotune); its pairs are exact like every other call's. */
   inline bool iteratePairwise(CollisionFunctor *f);
+  static constexpr size_t kTuneMinObjects = 16384;
 
   /* Simulation::deleteBurnUps (Simulation.cpp:561-596): drop objects below the altitude. */
   size_t deleteBurnUps(double minAltitude) {
@@ -307,6 +312,7 @@ class Container {
   bool host_stale_ = false;
   size_t appended_ = 0;
   double applied_dt_ = -1.0, applied_t0_ = 0.0;  // the configuration the context holds
+  bool tuned_ = false;
 
   void upload(size_t b, size_t e, int (*entry)(ladds_ctx *, const ladds_objects *), const char *what) {
     const size_t n = e - b;
@@ -408,6 +414,12 @@ inline bool Container::iteratePairwise(CollisionFunctor *f) {
   cfg_.evasion_cutoff_km = f->evasion_;
   cfg_.threshold_mode = LADDS_THRESHOLD_RADIUS_SCALED;
   applyConfig();
+  bool tuning = false;
+  if (cc_.tune && cc_.scanVariant != 1 && !tuned_ && ladds_num_objects(ctx_) >= kTuneMinObjects) {
+    ladds_tuning t{};
+    check(ladds_autotune(ctx_, f->dt_, 2, &t), ctx_, "ladds_autotune");
+    tuned_ = tuning = true;
+  }
   size_t n = 0, ne = 0;
   std::vector<ladds_pair> hits(4096), evaded(4096);
   int rc = ladds_conjunctions(ctx_, f->dt_, hits.data(), hits.size(), &n, evaded.data(), evaded.size(), &ne);
@@ -441,7 +453,7 @@ inline bool Container::iteratePairwise(CollisionFunctor *f) {
   };
   convert(hits, f->collisions_);
   convert(evaded, f->evaded_);
-  return false;
+  return tuning;
 }
 
 /* DatasetReader::readDataset (src/ladds/io/DatasetReader.cpp:15-50): a CSV file with one

@@ -464,6 +464,14 @@ class Engine:
                 "workgroups_per_cu": t.workgroups_per_cu, "scan_us": t.scan_us, "default_us": t.default_us,
                 "tried": t.tried}
 
+    def set_tuning(self, tuning: dict):
+        """Apply a shape returned by ``autotune`` (``ladds_set_tuning``) without timing anything."""
+        t = _abi.Tuning()
+        for k in ("cell_size_km", "plane_width_km", "plane_bits", "pair_threads", "tile_size", "chunks",
+                  "workgroups_per_cu"):
+            setattr(t, k, tuning[k])
+        self._check(lib.ladds_set_tuning(self._h, C.byref(t)), "set_tuning")
+
     # -- measurement ----------------------------------------------------
     def set_profiling(self, enable: bool):
         self._check(lib.ladds_set_profiling(self._h, int(bool(enable))), "set_profiling")

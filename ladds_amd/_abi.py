@@ -184,6 +184,7 @@ _SIGNATURES = {
     "ladds_clear_log": (C.c_int, [_ctx]),
     "ladds_delete_burnups": (C.c_int, [_ctx, C.c_double, C.POINTER(C.c_size_t)]),
     "ladds_autotune": (C.c_int, [_ctx, C.c_double, C.c_int, C.POINTER(Tuning)]),
+    "ladds_set_tuning": (C.c_int, [_ctx, C.POINTER(Tuning)]),
     "ladds_set_profiling": (C.c_int, [_ctx, C.c_int]),
     "ladds_kernel_stats": (C.c_int, [_ctx, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "ladds_synchronize": (C.c_int, [_ctx]),

@@ -119,6 +119,10 @@ struct ladds_ctx {
   // id exist (`protected_objects`), ladds_simulate keeps the count on the device
   // (`d_spawn`) and resets the parent ids there; otherwise the count is kept on the host
   // from the latest iteration with collisions.
+  // `slot` holds the plane-tile bucket keys of the current state and grid (written by the
+  // last ladds_propagate step), so a detection can skip k_bin_keys.
+  bool keys_valid = false;
+
   uint64_t quiet = 0;  // from 0 at ladds_create, as Simulation.h:227; continued across calls
   bool protected_objects = false;
   SpawnState *d_spawn = nullptr;
@@ -283,6 +287,7 @@ hipError_t alloc_objects(DeviceObjects &o, size_t cap) {
 // x-planes for the x-extent of the population (no aliasing of x-slabs), about one object
 // per bucket within a plane, and an LDS capacity sized from the largest plane pair.
 int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
+  ctx->keys_valid = false;
   const bool tiles = ctx->variant == kVariantPlaneTiles;
   const double radius = search_radius(ctx->form, ctx->cfg.cutoff_km);
   const double min_cell = (tiles ? 1.0 : 2.0) * radius * (1.0 + 1e-9);
@@ -389,7 +394,6 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   }
   tp.plane_bits = g.by + g.bz;
   tp.nplanes = 1u << g.bx;
-  tp.lds_header_words = pair_tiles_header_words(tp);
   // ladds_config.block_size overrides; by default FP64 records (32 B) leave room for one
   // workgroup per CU, which then needs 16 waves, while the split form fits several.
   const int bs = ctx->cfg.block_size;
@@ -398,6 +402,7 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
                         : (tp.nscen > 1 ? 128u : (ctx->form == kFormFp64 ? 1024u : 256u));
   if (ctx->tune_threads == 128 || ctx->tune_threads == 256 || ctx->tune_threads == 512 || ctx->tune_threads == 1024)
     tp.pair_threads = (uint32_t)ctx->tune_threads;
+  tp.lds_header_words = pair_tiles_header_words(tp);  // depends on the workgroup size
   const size_t rec = (ctx->form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
   size_t pair_max = 0;
   if (!plane_count.empty()) {
@@ -737,6 +742,7 @@ int compact_if_needed(ladds_ctx *ctx) {
       HIPCHK(ctx, hipMemcpy(act.data(), ctx->d.activity, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost));
     }
     std::swap(ctx->d, ctx->d_alt);
+    ctx->keys_valid = false;
     ctx->d.n = alive;
     ctx->d_alt.n = 0;
     ctx->n = alive;
@@ -1184,14 +1190,20 @@ int ladds_propagate(ladds_ctx *ctx, int nsteps) {
     const int rc = alloc_env(ctx);
     if (rc != LADDS_OK) return rc;
   }
+  // the last launch also writes the bucket keys of the final state (plane-tile scan), so a
+  // detection that follows (iteratePairwise after integrate) skips its binning pass
+  const bool tiles = ctx->variant == kVariantPlaneTiles;
   for (int seg0 = 0; seg0 < nsteps; seg0 += kEnvSegment) {
     const int seg = std::min(kEnvSegment, nsteps - seg0);
     int rc = stage_env(ctx, 0, seg);
-    for (int s = 0; s < seg && rc == LADDS_OK; s += kChunk)
-      rc = launch_propagate_steps(ctx, (size_t)s, std::min(kChunk, seg - s), false, 0.0);
+    for (int s = 0; s < seg && rc == LADDS_OK; s += kChunk) {
+      const int k = std::min(kChunk, seg - s);
+      rc = launch_propagate_steps(ctx, (size_t)s, k, tiles && seg0 + s + k == nsteps, 0.0);
+    }
     if (rc != LADDS_OK) return rc;
     ctx->step_count += (uint64_t)seg;
   }
+  ctx->keys_valid = tiles;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return LADDS_OK;
 }
@@ -1200,7 +1212,7 @@ static int run_detection_sync(ladds_ctx *ctx, double dt_interp, bool digest, Sca
                               uint64_t *entries) {
   HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
-  int rc = launch_detection(ctx, false, dt_interp, digest, 0);
+  int rc = launch_detection(ctx, ctx->keys_valid, dt_interp, digest, 0);
   if (rc != LADDS_OK) return rc;
   unsigned long long cnt = 0;
   HIPCHK(ctx, hipMemcpyAsync(out_counters, ctx->counters, sizeof(ScanCounters), hipMemcpyDeviceToHost, ctx->stream));
@@ -1363,6 +1375,32 @@ int ensure_xbuf(ladds_ctx *ctx, size_t bytes) {
 
 extern "C" {
 
+int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *t) {
+  if (!ctx || !t) return LADDS_E_INVALID;
+  if (ctx->variant != kVariantPlaneTiles) return ctx->fail(LADDS_E_STATE, "tuning covers the plane-tile scan");
+  const double radius = search_radius(ctx->form, ctx->cfg.cutoff_km);
+  const bool ok = t->cell_size_km >= radius * (1.0 + 1e-9) && t->plane_width_km >= t->cell_size_km &&
+                  t->plane_bits >= 4 && t->plane_bits <= 12 &&
+                  (t->pair_threads == 128 || t->pair_threads == 256 || t->pair_threads == 512 ||
+                   t->pair_threads == 1024) &&
+                  (t->tile_size == 4096 || t->tile_size == 8192 || t->tile_size == 16384) && t->chunks >= 1 &&
+                  t->chunks <= 32 && t->workgroups_per_cu >= 0 && t->workgroups_per_cu <= 8;
+  if (!ok) return ctx->fail(LADDS_E_INVALID, "tuning values outside the tuner's range");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  ctx->tune_cell_km = t->cell_size_km;
+  ctx->tune_fx = t->plane_width_km / t->cell_size_km;
+  ctx->tune_pb = t->plane_bits;
+  ctx->tune_threads = t->pair_threads;
+  ctx->tune_tile = t->tile_size;
+  ctx->tune_chunks = t->chunks;
+  ctx->tune_wgs = t->workgroups_per_cu;
+  if (ctx->n == 0) return LADDS_OK;  // applied at the next ladds_set_objects
+  std::vector<double> rx(ctx->n);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(rx.data(), ctx->d.rx, sizeof(double) * ctx->n, hipMemcpyDeviceToHost));
+  return setup_grid(ctx, ctx->n, rx.data());
+}
+
 int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_t cap, size_t *n, ladds_pair *evaded,
                        size_t ecap, size_t *ne) {
   if (!ctx || !n || !ne || (cap && !hits) || (ecap && !evaded)) return LADDS_E_INVALID;
@@ -1377,7 +1415,7 @@ int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_
   for (int attempt = 0; attempt < 2; ++attempt) {
     HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
     HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
-    int rc = launch_detection(ctx, false, dt_interp, false, 0);
+    int rc = launch_detection(ctx, ctx->keys_valid, dt_interp, false, 0);
     if (rc != LADDS_OK) return rc;
     if (!ctx->h_log) {
       HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&ctx->h_log), sizeof(LoggedPair) * 4096, 0));
@@ -1635,6 +1673,7 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   const ScanCounters sc = ctx->h_rb->sc;
   const unsigned long long cnt = ctx->h_rb->log_count;
   ctx->log_nonempty = cnt != 0;
+  ctx->keys_valid = false;
   // spawn-protection count at the start of the next iteration (Simulation.cpp:399-402, 438)
   if (spawn) {
     ctx->quiet = ctx->h_rb->spawn.quiet;
