@@ -432,7 +432,8 @@ def main():
     print(f"# population {a.objects} generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
     forms = {"fp64": la.FP64, "split": la.SPLIT}
     res = {}
-    pinned = json.loads(Path(a.tuning).read_text() if a.tuning and Path(a.tuning).exists() else a.tuning or "{}")
+    text = (a.tuning or "{}").strip()
+    pinned = json.loads(text if text.startswith("{") else Path(text).read_text())
     for name in a.forms.split(","):
         res[name] = run_form(la, pop, forms[name], a.steps, a.warmup, a.profile_steps, not a.no_autotune,
                              pinned.get(name))

@@ -99,7 +99,7 @@ typedef struct ladds_config {
   double threshold_km;               /* absolute mode threshold                               */
   double cell_size_km;               /* 0: 2 * cutoff (minimum allowed)                        */
   int32_t scan_variant;              /* 0: auto/autotuned, 1: global-memory scan, 2: LDS tiles */
-  int32_t block_size;                /* plane-tile pair kernel workgroup: 256/512/1024, else auto */
+  int32_t block_size;                /* plane-tile pair kernel workgroup: 128/256/384/512/1024, else auto */
   int32_t device;                    /* HIP device ordinal                                    */
   int32_t pair_output;               /* ladds_simulate: LADDS_PAIRS_LIST | LADDS_PAIRS_DIGEST  */
   uint64_t log_capacity;             /* pair log entries kept on device by ladds_simulate     */

@@ -397,11 +397,10 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   // ladds_config.block_size overrides; by default FP64 records (32 B) leave room for one
   // workgroup per CU, which then needs 16 waves, while the split form fits several.
   const int bs = ctx->cfg.block_size;
-  tp.pair_threads = (bs == 128 || bs == 256 || bs == 512 || bs == 1024)
+  tp.pair_threads = valid_pair_threads(bs)
                         ? (uint32_t)bs
                         : (tp.nscen > 1 ? 128u : (ctx->form == kFormFp64 ? 1024u : 256u));
-  if (ctx->tune_threads == 128 || ctx->tune_threads == 256 || ctx->tune_threads == 512 || ctx->tune_threads == 1024)
-    tp.pair_threads = (uint32_t)ctx->tune_threads;
+  if (valid_pair_threads(ctx->tune_threads)) tp.pair_threads = (uint32_t)ctx->tune_threads;
   tp.lds_header_words = pair_tiles_header_words(tp);  // depends on the workgroup size
   const size_t rec = (ctx->form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
   size_t pair_max = 0;
@@ -430,7 +429,10 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
                                         "use fewer objects per context or a larger partition tile");
     const size_t fit = (kPairTileLdsBytes - hdr) / rec;
     const size_t need = std::min(fit, std::max<size_t>(1024, (pair_max + 63) / 64 * 64));
-    const size_t wave_cap = std::max<size_t>(1, 16 / std::max<uint32_t>(1, tp.pair_threads / 64));
+    // workgroups per CU the kernel's registers allow (16 waves per CU when unknown)
+    const int by_regs = pair_tiles_groups_per_cu(ctx->form, tp.pair_threads);
+    const size_t wave_cap = by_regs > 0 ? (size_t)by_regs
+                                        : std::max<size_t>(1, 16 / std::max<uint32_t>(1, tp.pair_threads / 64));
     size_t wgs = std::min(wave_cap, kPairTileLdsBytes / (hdr + need * rec + kLdsMargin));
     if (ctx->tune_wgs >= 1 && ctx->tune_wgs <= 8) wgs = (size_t)ctx->tune_wgs;
     wgs = std::max<size_t>(1, wgs);
@@ -1291,7 +1293,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   // plane (load balance of dense planes), workgroup size, cell size, plane shape, tile,
   // LDS share (workgroups per CU).
   const int chunk_opts[] = {1, 2, 4, 8, 16, 32};
-  const int thread_opts[] = {128, 256, 512, 1024};
+  const int thread_opts[] = {128, 256, 384, 512, 1024};
   const double cell_opts[] = {1.0, 1.25, 1.5, 2.0};
   const int tile_opts[] = {4096, 8192, 16384};
   const double fx_opts[] = {1.0, 1.5, 2.0, 3.0};
@@ -1381,8 +1383,7 @@ int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *t) {
   const double radius = search_radius(ctx->form, ctx->cfg.cutoff_km);
   const bool ok = t->cell_size_km >= radius * (1.0 + 1e-9) && t->plane_width_km >= t->cell_size_km &&
                   t->plane_bits >= 4 && t->plane_bits <= 12 &&
-                  (t->pair_threads == 128 || t->pair_threads == 256 || t->pair_threads == 512 ||
-                   t->pair_threads == 1024) &&
+                  valid_pair_threads(t->pair_threads) &&
                   (t->tile_size == 4096 || t->tile_size == 8192 || t->tile_size == 16384) && t->chunks >= 1 &&
                   t->chunks <= 32 && t->workgroups_per_cu >= 0 && t->workgroups_per_cu <= 8;
   if (!ok) return ctx->fail(LADDS_E_INVALID, "tuning values outside the tuner's range");

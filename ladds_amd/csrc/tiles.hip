@@ -276,7 +276,9 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
             c0 = c0 && o0.idx > me.idx;
             c1 = c1 && o1.idx > me.idx;
           }
-          if (c0 || c1) {
+          // candidates are rare (~1 in 10^4 tests at 1M objects): the exact predicate's
+          // path is laid out as the cold one
+          if (__builtin_expect(c0 || c1, 0)) {
 #pragma unroll 1
             for (int u = 0; u < 2; ++u)
               if (u ? c1 : c0) handle_candidate<DIGEST>(a.sink, me.idx, u ? o1.idx : o0.idx, acc, scen);
@@ -460,9 +462,16 @@ __device__ __forceinline__ uint32_t xcd_local_item(uint32_t b, uint32_t g) {
   return (slot / kXcdRun) * (kXcds * kXcdRun) + xcd * kXcdRun + slot % kXcdRun;
 }
 
+// Occupancy target of the pair kernel: 5 waves per SIMD lets the register allocator keep
+// the split form at 77 VGPRs (list mode), 6 waves per SIMD, instead of 103 (4 waves) with
+// no spills; the FP64 form spills under it below 512 threads and keeps the default there.
+template <int FORM, int THREADS>
+constexpr int kPairWavesPerEu = (FORM == kFormSplit || THREADS >= 512) ? 5 : 1;
+
 // One workgroup per plane pair (grid nplanes * nscen * nchunk).
 template <int FORM, bool DIGEST, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_pair_tiles(PairTileArgs a) {
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kPairWavesPerEu<FORM, THREADS>)))
+void k_pair_tiles(PairTileArgs a) {
   extern __shared__ uint32_t lds[];
   ScanAcc acc;
   pair_plane<FORM, DIGEST, THREADS>(a, lds, xcd_local_item(blockIdx.x, gridDim.x), acc);
@@ -539,6 +548,7 @@ hipError_t configure_tile_kernels() {
   const int max_lds = 160 * 1024;
   hipError_t e = set_pair_lds_limit<128>(max_lds);
   if (e == hipSuccess) e = set_pair_lds_limit<256>(max_lds);
+  if (e == hipSuccess) e = set_pair_lds_limit<384>(max_lds);
   if (e == hipSuccess) e = set_pair_lds_limit<512>(max_lds);
   if (e == hipSuccess) e = set_pair_lds_limit<1024>(max_lds);
   auto set = [&](const void *f) {
@@ -551,6 +561,32 @@ hipError_t configure_tile_kernels() {
   set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 8>));
   set(reinterpret_cast<const void *>(&k_tile_partition<RecordF32, 16>));
   return e;
+}
+
+template <int THREADS>
+int pair_groups_per_cu_as(int form) {
+  int n = 0;
+  const void *f = form == kFormFp64 ? reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS>)
+                                    : reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, THREADS, 0) != hipSuccess) return 0;
+  return n;
+}
+
+int pair_tiles_groups_per_cu(int form, uint32_t threads) {
+  switch (threads) {
+    case 128:
+      return pair_groups_per_cu_as<128>(form);
+    case 256:
+      return pair_groups_per_cu_as<256>(form);
+    case 384:
+      return pair_groups_per_cu_as<384>(form);
+    case 512:
+      return pair_groups_per_cu_as<512>(form);
+    case 1024:
+      return pair_groups_per_cu_as<1024>(form);
+    default:
+      return 0;
+  }
 }
 
 uint32_t pair_tiles_header_words(const TileParams &tp) {
@@ -596,6 +632,9 @@ hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, cons
       break;
     case 512:
       launch_pair_tiles_as<512>(form, digest, a, s);
+      break;
+    case 384:
+      launch_pair_tiles_as<384>(form, digest, a, s);
       break;
     case 1024:
       launch_pair_tiles_as<1024>(form, digest, a, s);

@@ -457,3 +457,30 @@ def test_lds_and_arena_planes_in_one_scan(precision):
     _pair_equal(ol.sort_pairs(hits), oh)
     _pair_equal(ol.sort_pairs(ev), oe)
     assert cnt == oh.shape[0] and dig == ol.digest(oh)
+
+
+@pytest.mark.parametrize("threads", [128, 256, 384, 512, 1024])
+@pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
+def test_every_workgroup_size_keeps_the_pair_set(threads, precision):
+    """The plane-tile pair kernel at each workgroup size (each has its own register budget
+    and its own split between the register-resident and the two-pass gather): a LEO
+    population with planted pairs plus a 2,500-object cluster, whose plane pair takes the
+    two-pass gather below 512 threads.  Pair list and digest equal the oracle's."""
+    pop = la.generate_population(1, 0x1ADD5 + 17, 40000, planted_pairs=300)
+    rng = np.random.default_rng(17)
+    c = np.array([pop.rx[0], pop.ry[0], pop.rz[0]])
+    for i in range(1, 2500):
+        pop.rx[i], pop.ry[i], pop.rz[i] = c + rng.uniform(-4.0, 4.0, 3)
+        pop.vx[i], pop.vy[i], pop.vz[i] = pop.vx[0], pop.vy[0], pop.vz[0] + rng.uniform(-1e-3, 1e-3)
+    with engine(dt=10.0, precision=precision, cutoff_km=10.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
+                threshold_km=1.0, block_size=threads) as eng:
+        eng.set_objects(pop)
+        hits, ev = eng.conjunctions()
+        cnt, dig, ecnt, edig = eng.conjunction_digest()
+    oh, oe = ol.pairs(_oracle_objects(pop.r(), pop.v(), pop),
+                      ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0))
+    assert oh.shape[0] > 1000
+    _pair_equal(ol.sort_pairs(hits), oh)
+    _pair_equal(ol.sort_pairs(ev), oe)
+    assert cnt == oh.shape[0] and dig == ol.digest(oh)
+    assert ecnt == oe.shape[0] and edig == ol.digest(oe)
