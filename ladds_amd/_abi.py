@@ -109,6 +109,7 @@ class Tuning(C.Structure):
         ("chunks", C.c_int32),
         ("workgroups_per_cu", C.c_int32),
         ("plane_width_km", C.c_double),
+        ("plane_order", C.c_int32),
     ]
 
 

@@ -64,6 +64,8 @@ struct TileParams {
   // the plane itself (grid: nchunk workgroups per plane, the unused ones exit).
   uint32_t nchunk;
   uint32_t arena_cap;          // records of the global scratch arena
+  // One population: the planes in dispatch order (nplanes entries), or null for plane order.
+  const uint32_t *plane_order;
 };
 
 // Sorted record of the plain FP64 form and of the split form: the position the

@@ -474,7 +474,9 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kPairWa
 void k_pair_tiles(PairTileArgs a) {
   extern __shared__ uint32_t lds[];
   ScanAcc acc;
-  pair_plane<FORM, DIGEST, THREADS>(a, lds, xcd_local_item(blockIdx.x, gridDim.x), acc);
+  uint32_t item = xcd_local_item(blockIdx.x, gridDim.x);
+  if (a.tp.plane_order) item = a.tp.plane_order[item / a.tp.nchunk] * a.tp.nchunk + item % a.tp.nchunk;
+  pair_plane<FORM, DIGEST, THREADS>(a, lds, item, acc);
   flush_counters<DIGEST>(a.sink, acc);
 }
 
