@@ -56,8 +56,10 @@ BYTES = {
 
 
 # HBM bytes per launch at 1,048,576 objects (read + write), measured with rocprofv3
-# FETCH_SIZE / WRITE_SIZE scaled by a 256 MiB copy kernel, default plane-tile shape; the
-# runs and the commands are in profiles/hbm_traffic_1M_full_model.md.
+# FETCH_SIZE / WRITE_SIZE scaled by a 256 MiB copy kernel, on the plane-tile shape the
+# autotuner picks for this workload (FP64: 512-thread pair workgroups, 4,096-object tiles;
+# split: 256 threads, 8,192-object tiles); the runs and the commands are in
+# profiles/hbm_traffic_1M_full_model.md.
 # FP64 and FP32 VALU flops per object-step of the propagation kernel (full force model),
 # from rocprofv3 SQ_INSTS_VALU_FLOPS_{FP64,FP32} (+ the _TRANS counters), per wavefront
 # instruction and lane, on the 1M-object workload.
@@ -69,12 +71,12 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64
 FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32 (non-packed issue is half of this)
 
 MEASURED_TRAFFIC_1M = {
-    ("propagate", "fp64"): (71.46 + 58.72) * 1e6,
-    ("propagate", "split"): (67.27 + 58.72) * 1e6,
-    ("tile_partition", "fp64"): (29.44 + 48.08) * 1e6,
-    ("tile_partition", "split"): (29.44 + 17.83) * 1e6,
-    ("pair_tiles", "fp64"): (50.10 + 0.06) * 1e6,
-    ("pair_tiles", "split"): (31.74 + 0.06) * 1e6,
+    ("propagate", "fp64"): (71.47 + 54.65) * 1e6,
+    ("propagate", "split"): (67.27 + 54.65) * 1e6,
+    ("tile_partition", "fp64"): (29.43 + 35.94) * 1e6,
+    ("tile_partition", "split"): (29.45 + 17.83) * 1e6,
+    ("pair_tiles", "fp64"): (56.09 + 0.06) * 1e6,
+    ("pair_tiles", "split"): (31.76 + 0.06) * 1e6,
 }
 
 
@@ -103,6 +105,9 @@ def parse():
     p.add_argument("--tuning", metavar="JSON",
                    help="apply these plane-tile shapes instead of autotuning: a JSON object per form, as the "
                         "'tuning' field of an earlier run's line (profiler runs then measure the tuned shape)")
+    p.add_argument("--profile-mode", choices=("overlapped", "serial"), default="overlapped",
+                   help="profiled pass: the overlapped loop (default), or one kernel at a time (propagate one "
+                        "step, then scan its pairs)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="first copy MIB MiB device-to-device 3 times (a plain copy kernel of known size, "
                         "for scaling hardware byte counters), then run the bench")
@@ -126,7 +131,7 @@ def make_engine(la, precision):
     return la.Engine(cfg)
 
 
-def run_form(la, pop, precision, steps, warmup, profile_steps, autotune=True, pinned=None):
+def run_form(la, pop, precision, steps, warmup, profile_steps, autotune=True, pinned=None, serial=False):
     eng = make_engine(la, precision)
     eng.set_objects(pop)
     # plane-tile shape picked by timing the scan on the initial state (outside the timed region),
@@ -154,7 +159,14 @@ def run_form(la, pop, precision, steps, warmup, profile_steps, autotune=True, pi
     eng.set_objects(pop)
     eng.set_time(0.0)
     eng.set_profiling(True)
-    eng.simulate(profile_steps, first_iteration=0, tpcd=1, min_altitude_km=MIN_ALT)
+    if serial:
+        # one kernel at a time: propagation step, then the scan of its state (same kernels as
+        # the loop's, without the other stream's work beside them)
+        for _ in range(profile_steps):
+            eng.propagate(1)
+            eng.conjunctions()
+    else:
+        eng.simulate(profile_steps, first_iteration=0, tpcd=1, min_altitude_km=MIN_ALT)
     kernels = {}
     for name in ("propagate", "bin_keys", "tile_partition", "pair_tiles", "bin_reduce", "bin_scan", "bin_scatter",
                  "pair_scan"):
@@ -436,7 +448,7 @@ def main():
     pinned = json.loads(text if text.startswith("{") else Path(text).read_text())
     for name in a.forms.split(","):
         res[name] = run_form(la, pop, forms[name], a.steps, a.warmup, a.profile_steps, not a.no_autotune,
-                             pinned.get(name))
+                             pinned.get(name), a.profile_mode == "serial")
         r = res[name]
         print(f"# {name}: {a.steps} steps in {r['elapsed']:.3f} s, {a.objects * a.steps / r['elapsed']:.4g} "
               f"object-steps/s, collisions {r['stats']['collisions']}, candidates {r['stats']['candidates']}, "
