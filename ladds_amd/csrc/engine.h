@@ -64,6 +64,11 @@ struct TileParams {
   // the plane itself (grid: nchunk workgroups per plane, the unused ones exit).
   uint32_t nchunk;
   uint32_t arena_cap;          // records of the global scratch arena
+  // 1: the arena holds arena_cap slots of kArenaSlotBytes (sorted records) followed by
+  // arena_cap predicate records (kFatRecordBytes each) at the same positions, and the pair
+  // kernel's arena planes read the predicate's fields from them (set when plane pairs
+  // exceeded the LDS capacity at set-up: dense, clustered populations)
+  uint32_t arena_fat;
   // One population: the planes in dispatch order (nplanes entries), or null for plane order.
   const uint32_t *plane_order;
 };

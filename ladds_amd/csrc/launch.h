@@ -51,6 +51,10 @@ constexpr uint32_t kPairTileLdsBytes = 160 * 1024;
 // LDS reserved per pair-kernel workgroup beyond its request (allocation granularity).
 constexpr uint32_t kLdsMargin = 1024;
 constexpr uint32_t kPartitionTile = 8192;
+// Global scratch arena of the pair scan: per record slot a sorted record (32 bytes
+// reserved, the FP64 record's size) and the fields the exact predicate reads (80 bytes).
+constexpr size_t kArenaSlotBytes = 32;
+constexpr size_t kFatRecordBytes = 80;
 
 // Search radius of the pair scan: the cutoff in the FP64 form; in the split form the
 // FP32 prefilter radius, inflated so that FP32 rounding of positions (< 2^-24 |x| per

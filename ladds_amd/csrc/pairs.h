@@ -30,25 +30,81 @@ __device__ __forceinline__ double dot3(double a0, double a1, double a2, double b
   return exact_dot3(a0, a1, a2, b0, b1, b2);
 }
 
-// Returns 0 (no report), 1 (collision) or 2 (evaded); i < j are object indices, evaluated
-// as AoSFunctor(i, j).
-__device__ __forceinline__ int collision_functor(const DeviceObjects &o, uint32_t i, uint32_t j,
+// What the predicate reads of one object: object `i` of the object arrays, or a copy of
+// its fields stored beside the sorted records of a dense plane pair (FatRecord).
+struct SoAObject {
+  const DeviceObjects &o;
+  uint32_t i;
+  __device__ __forceinline__ int activity() const { return o.activity[i]; }
+  __device__ __forceinline__ double radius() const { return o.radius_m[i]; }
+  __device__ __forceinline__ uint64_t parent() const { return o.parent[i]; }
+  __device__ __forceinline__ uint64_t id() const { return o.id[i]; }
+  __device__ __forceinline__ double rx() const { return o.rx[i]; }
+  __device__ __forceinline__ double ry() const { return o.ry[i]; }
+  __device__ __forceinline__ double rz() const { return o.rz[i]; }
+  __device__ __forceinline__ double vx() const { return o.vx[i]; }
+  __device__ __forceinline__ double vy() const { return o.vy[i]; }
+  __device__ __forceinline__ double vz() const { return o.vz[i]; }
+};
+
+struct alignas(16) FatRecord {
+  double r[3], v[3];
+  double radius_m;
+  uint64_t parent, id;
+  int32_t activity;
+  uint32_t pad;
+};
+
+__device__ __forceinline__ FatRecord fat_record_of(const DeviceObjects &o, uint32_t i) {
+  FatRecord f;
+  f.r[0] = o.rx[i];
+  f.r[1] = o.ry[i];
+  f.r[2] = o.rz[i];
+  f.v[0] = o.vx[i];
+  f.v[1] = o.vy[i];
+  f.v[2] = o.vz[i];
+  f.radius_m = o.radius_m[i];
+  f.parent = o.parent[i];
+  f.id = o.id[i];
+  f.activity = o.activity[i];
+  f.pad = 0;
+  return f;
+}
+
+struct FatObject {
+  const FatRecord &f;
+  __device__ __forceinline__ int activity() const { return f.activity; }
+  __device__ __forceinline__ double radius() const { return f.radius_m; }
+  __device__ __forceinline__ uint64_t parent() const { return f.parent; }
+  __device__ __forceinline__ uint64_t id() const { return f.id; }
+  __device__ __forceinline__ double rx() const { return f.r[0]; }
+  __device__ __forceinline__ double ry() const { return f.r[1]; }
+  __device__ __forceinline__ double rz() const { return f.r[2]; }
+  __device__ __forceinline__ double vx() const { return f.v[0]; }
+  __device__ __forceinline__ double vy() const { return f.v[1]; }
+  __device__ __forceinline__ double vz() const { return f.v[2]; }
+};
+
+// Returns 0 (no report), 1 (collision) or 2 (evaded); objects `a` (index i) and `b`
+// (index j), i < j, evaluated as AoSFunctor(i, j).
+template <typename OBJ>
+__device__ __forceinline__ int collision_functor(const OBJ &a, const OBJ &b, uint32_t i, uint32_t j,
                                               const FunctorParams &fp, PairRecord &out) {
 #pragma clang fp contract(off)
-  const int ai = o.activity[i], aj = o.activity[j];
+  const int ai = a.activity(), aj = b.activity();
   if (ai < 0 || aj < 0) return 0;  // deleted (dummy) objects, CollisionFunctor.cpp:50-52
-  const double Ri = o.radius_m[i], Rj = o.radius_m[j];
+  const double Ri = a.radius(), Rj = b.radius();
   const bool evaded = (ai != 0 && aj != 0) || (ai != 0 && Rj >= fp.min_det_radius) ||
                       (aj != 0 && Ri >= fp.min_det_radius);
-  const uint64_t pi = o.parent[i], pj = o.parent[j];
+  const uint64_t pi = a.parent(), pj = b.parent();
   if (pi != ~0ull && pi == pj) return 0;
-  const double rix = o.rx[i], riy = o.ry[i], riz = o.rz[i];
-  const double rjx = o.rx[j], rjy = o.ry[j], rjz = o.rz[j];
+  const double rix = a.rx(), riy = a.ry(), riz = a.rz();
+  const double rjx = b.rx(), rjy = b.ry(), rjz = b.rz();
   const double dx = rix - rjx, dy = riy - rjy, dz = riz - rjz;
   const double d0 = dot3(dx, dy, dz, dx, dy, dz);
   if (d0 > fp.cutoff2) return 0;
-  const double vix = o.vx[i], viy = o.vy[i], viz = o.vz[i];
-  const double vjx = o.vx[j], vjy = o.vy[j], vjz = o.vz[j];
+  const double vix = a.vx(), viy = a.vy(), viz = a.vz();
+  const double vjx = b.vx(), vjy = b.vy(), vjz = b.vz();
   const double dt = fp.dt;
   // DistanceApproximation.cpp:25-55
   const double oix = rix - vix * dt, oiy = riy - viy * dt, oiz = riz - viz * dt;
@@ -81,7 +137,7 @@ __device__ __forceinline__ int collision_functor(const DeviceObjects &o, uint32_
   out.mid[1] = p2y + dry * 0.5;
   out.mid[2] = p2z + drz * 0.5;
   out.d2 = dl;
-  const uint64_t idi = o.id[i], idj = o.id[j];
+  const uint64_t idi = a.id(), idj = b.id();
   if (idi < idj) {
     out.id_a = idi;
     out.id_b = idj;
@@ -123,13 +179,10 @@ struct ScanAcc {
   unsigned long long cand = 0, hit = 0, ev = 0, d0 = 0, d1 = 0, e0 = 0, e1 = 0;
 };
 
+// A predicate result to the sink: digest sums, or the log (wavefront compaction).
 template <bool DIGEST>
-__device__ __forceinline__ void handle_candidate(const PairSink &a, uint32_t oi, uint32_t oj, ScanAcc &acc,
-                                                 uint32_t scenario = 0u) {
-  const uint32_t i = oi < oj ? oi : oj;
-  const uint32_t j = oi < oj ? oj : oi;
-  PairRecord r;
-  const int kind = collision_functor(a.o, i, j, a.fp, r);
+__device__ __forceinline__ void sink_result(const PairSink &a, int kind, const PairRecord &r, ScanAcc &acc,
+                                            uint32_t scenario) {
   acc.cand += 1;
   if constexpr (DIGEST) {
     if (kind != 0) {
@@ -176,6 +229,29 @@ __device__ __forceinline__ void handle_candidate(const PairSink &a, uint32_t oi,
       }
     }
   }
+}
+
+// The predicate of objects oi and oj read from the object arrays.
+template <bool DIGEST>
+__device__ __forceinline__ void handle_candidate(const PairSink &a, uint32_t oi, uint32_t oj, ScanAcc &acc,
+                                                 uint32_t scenario = 0u) {
+  const uint32_t i = oi < oj ? oi : oj;
+  const uint32_t j = oi < oj ? oj : oi;
+  PairRecord r;
+  const int kind = collision_functor(SoAObject{a.o, i}, SoAObject{a.o, j}, i, j, a.fp, r);
+  sink_result<DIGEST>(a, kind, r, acc, scenario);
+}
+
+// The same predicate with both objects' fields read from their FatRecords fi (object oi)
+// and fj (object oj).
+template <bool DIGEST>
+__device__ __forceinline__ void handle_candidate(const PairSink &a, const FatRecord &fi, const FatRecord &fj,
+                                                 uint32_t oi, uint32_t oj, ScanAcc &acc, uint32_t scenario) {
+  const bool lo = oi < oj;
+  PairRecord r;
+  const int kind = collision_functor(FatObject{lo ? fi : fj}, FatObject{lo ? fj : fi}, lo ? oi : oj, lo ? oj : oi,
+                                     a.fp, r);
+  sink_result<DIGEST>(a, kind, r, acc, scenario);
 }
 
 // Flush the per-lane counters: one atomic per wavefront.  All lanes of the wave must call it.

@@ -223,11 +223,13 @@ __device__ __forceinline__ bool within(const RecOf<FORM> &a, const RecOf<FORM> &
 // next), so bucket b spans [cur[b-1], cur[b]).  The rows are
 // walked by one loop so that the candidate path (the exact predicate and the wavefront
 // compaction) exists once in the code.
+// `fat` (or null): the predicate records at the sorted positions, for plane pairs sorted in
+// the global arena.
 template <int FORM, bool DIGEST, bool CHUNKED>
 __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FORM> *sorted,
                                            const uint32_t *cur_own, const uint32_t *cur_halo, uint32_t n_own,
                                            uint32_t threads, uint32_t scen, uint32_t chunk, uint32_t nchunk,
-                                           ScanAcc &acc) {
+                                           const FatRecord *fat, ScanAcc &acc) {
   const GridParams &g = a.g;
   const uint32_t my = (1u << g.by) - 1u, mz = (1u << g.bz) - 1u;
   const uint32_t sy = g.bz;
@@ -280,8 +282,14 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
           // path is laid out as the cold one
           if (__builtin_expect(c0 || c1, 0)) {
 #pragma unroll 1
-            for (int u = 0; u < 2; ++u)
-              if (u ? c1 : c0) handle_candidate<DIGEST>(a.sink, me.idx, u ? o1.idx : o0.idx, acc, scen);
+            for (int u = 0; u < 2; ++u) {
+              if (!(u ? c1 : c0)) continue;
+              const uint32_t oj = u ? o1.idx : o0.idx;
+              if (fat)
+                handle_candidate<DIGEST>(a.sink, fat[s], fat[q + (uint32_t)u], me.idx, oj, acc, scen);
+              else
+                handle_candidate<DIGEST>(a.sink, me.idx, oj, acc, scen);
+            }
           }
         }
         q = max(lo1, from);
@@ -293,7 +301,10 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
 
 // One work item of the pair scan: plane pair `wid` = (scenario, plane, chunk), sorted in
 // LDS up to tp.cap_items records, beyond that in the global scratch arena.
-template <int FORM, bool DIGEST, int THREADS>
+// FAT: plane pairs sorted in the arena carry the predicate's fields beside their records
+// (instantiated only for launches whose shape left plane pairs beyond the LDS, so that the
+// common all-in-LDS case keeps the smaller register footprint).
+template <int FORM, bool DIGEST, int THREADS, bool FAT>
 __device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds, uint32_t wid, ScanAcc &acc) {
   constexpr int kPairRegItems = kPairRegCapacity / THREADS < 8 ? kPairRegCapacity / THREADS : 8;
   using REC = RecOf<FORM>;
@@ -350,6 +361,11 @@ __device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds,
     __syncthreads();
     if (!in_lds && !misc[1]) return;  // no room: the plane is reported, never written out of bounds
     REC *sorted = in_lds ? lds_items : static_cast<REC *>(a.scratch) + misc[0];
+    // dense plane pairs sorted in the arena: the predicate's fields beside each record
+    FatRecord *fat = (FAT && !in_lds)
+                         ? reinterpret_cast<FatRecord *>(static_cast<char *>(a.scratch) +
+                                                         (size_t)tp.arena_cap * kArenaSlotBytes) + misc[0]
+                         : nullptr;
     // own records at [0, n_own), the next plane's at [n_own, total)
     // Each thread reads a block of consecutive records: one binary search for the
     // block's first record, then a forward walk over the segment boundaries.
@@ -434,17 +450,25 @@ __device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds,
         if (pass == 0) block_exclusive_scan(cur_own, 2u * L + 1u, wsum);
       }
     }
-    // cur[b] now holds the end of bucket b
+    // cur[b] now holds the end of bucket b.  Arena plane pairs: the predicate's fields of
+    // every sorted record, written in sorted order (the records are visible to the whole
+    // workgroup after the barrier above)
+    if (fat) {
+      for (uint32_t p = threadIdx.x; p < total; p += THREADS) fat[p] = fat_record_of(a.sink.o, sorted[p].idx);
+      __syncthreads();
+    }
     if (cp > 1u) {
       if (in_lds)
-        scan_plane<FORM, DIGEST, true>(a, lds_items, cur_own, cur_next, n_own, THREADS, scen, chunk, cp, acc);
+        scan_plane<FORM, DIGEST, true>(a, lds_items, cur_own, cur_next, n_own, THREADS, scen, chunk, cp, nullptr,
+                                       acc);
       else
-        scan_plane<FORM, DIGEST, true>(a, sorted, cur_own, cur_next, n_own, THREADS, scen, chunk, cp, acc);
+        scan_plane<FORM, DIGEST, true>(a, sorted, cur_own, cur_next, n_own, THREADS, scen, chunk, cp, fat, acc);
     } else {
       if (in_lds)
-        scan_plane<FORM, DIGEST, false>(a, lds_items, cur_own, cur_next, n_own, THREADS, scen, 0u, 1u, acc);
+        scan_plane<FORM, DIGEST, false>(a, lds_items, cur_own, cur_next, n_own, THREADS, scen, 0u, 1u, nullptr,
+                                        acc);
       else
-        scan_plane<FORM, DIGEST, false>(a, sorted, cur_own, cur_next, n_own, THREADS, scen, 0u, 1u, acc);
+        scan_plane<FORM, DIGEST, false>(a, sorted, cur_own, cur_next, n_own, THREADS, scen, 0u, 1u, fat, acc);
     }
   }
 }
@@ -468,15 +492,18 @@ __device__ __forceinline__ uint32_t xcd_local_item(uint32_t b, uint32_t g) {
 template <int FORM, int THREADS>
 constexpr int kPairWavesPerEu = (FORM == kFormSplit || THREADS >= 512) ? 5 : 1;
 
+static_assert(sizeof(FatRecord) == kFatRecordBytes, "predicate record size");
+static_assert(sizeof(RecordF64) <= kArenaSlotBytes && sizeof(RecordF32) <= kArenaSlotBytes, "arena slot");
+
 // One workgroup per plane pair (grid nplanes * nscen * nchunk).
-template <int FORM, bool DIGEST, int THREADS>
+template <int FORM, bool DIGEST, int THREADS, bool FAT>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kPairWavesPerEu<FORM, THREADS>)))
 void k_pair_tiles(PairTileArgs a) {
   extern __shared__ uint32_t lds[];
   ScanAcc acc;
   uint32_t item = xcd_local_item(blockIdx.x, gridDim.x);
   if (a.tp.plane_order) item = a.tp.plane_order[item / a.tp.nchunk] * a.tp.nchunk + item % a.tp.nchunk;
-  pair_plane<FORM, DIGEST, THREADS>(a, lds, item, acc);
+  pair_plane<FORM, DIGEST, THREADS, FAT>(a, lds, item, acc);
   flush_counters<DIGEST>(a.sink, acc);
 }
 
@@ -539,10 +566,14 @@ hipError_t set_pair_lds_limit(int max_lds) {
   auto set = [&](const void *f) {
     if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
   };
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, true, THREADS>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, true, THREADS>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS, false>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, true, THREADS, false>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS, false>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, true, THREADS, false>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS, true>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, true, THREADS, true>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS, true>));
+  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, true, THREADS, true>));
   return e;
 }
 
@@ -568,8 +599,8 @@ hipError_t configure_tile_kernels() {
 template <int THREADS>
 int pair_groups_per_cu_as(int form) {
   int n = 0;
-  const void *f = form == kFormFp64 ? reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS>)
-                                    : reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS>);
+  const void *f = form == kFormFp64 ? reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS, false>)
+                                    : reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS, false>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, THREADS, 0) != hipSuccess) return 0;
   return n;
 }
@@ -605,10 +636,17 @@ void launch_pair_tiles_as(int form, bool digest, const PairTileArgs &a, hipStrea
   const size_t shm = sizeof(uint32_t) * tp.lds_header_words + rec * tp.cap_items;
   const dim3 grid(tp.nplanes * tp.nscen * tp.nchunk), block(THREADS);
   const bool f64 = (form == kFormFp64);
-  if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false, THREADS>), grid, block, shm, s, a);
-  if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true, THREADS>), grid, block, shm, s, a);
-  if (!f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, false, THREADS>), grid, block, shm, s, a);
-  if (!f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, true, THREADS>), grid, block, shm, s, a);
+  if (tp.arena_fat) {
+    if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false, THREADS, true>), grid, block, shm, s, a);
+    if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true, THREADS, true>), grid, block, shm, s, a);
+    if (!f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, false, THREADS, true>), grid, block, shm, s, a);
+    if (!f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, true, THREADS, true>), grid, block, shm, s, a);
+  } else {
+    if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false, THREADS, false>), grid, block, shm, s, a);
+    if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true, THREADS, false>), grid, block, shm, s, a);
+    if (!f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, false, THREADS, false>), grid, block, shm, s, a);
+    if (!f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, true, THREADS, false>), grid, block, shm, s, a);
+  }
 }
 
 hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, const GridParams &g,

@@ -78,3 +78,28 @@ def test_noalloc_rejects_more_than_its_maximum():
     cfg = la.EngineConfig(cutoff_km=10.0).to_c()
     with pytest.raises(la.LaddsError, match="LADDS_NOALLOC_MAX_OBJECTS"):
         _noalloc(d, cfg, 10.0)
+
+
+@pytest.mark.parametrize("precision", [la.FP64, la.SPLIT], ids=["fp64", "split"])
+def test_noalloc_dense_plane_sorted_in_the_arena(precision):
+    """A 5,000-object cluster inside 4 km: its plane pair exceeds the pair kernel's LDS and
+    is sorted in the global arena.  The context path (whose set-up sees the dense plane and
+    stores the predicate's fields beside the arena records) and the no-allocation path (the
+    predicate gathers from the object arrays) give the same digest."""
+    pop = la.generate_population(1, 0x1ADD5 + 41, 30000, planted_pairs=100)
+    rng = np.random.default_rng(41)
+    c = np.array([pop.rx[0], pop.ry[0], pop.rz[0]])
+    for i in range(1, 5000):
+        pop.rx[i], pop.ry[i], pop.rz[i] = c + rng.uniform(-2.0, 2.0, 3)
+        pop.vx[i], pop.vy[i], pop.vz[i] = pop.vx[0], pop.vy[0], pop.vz[0] + rng.uniform(-1e-3, 1e-3)
+    ecfg = la.EngineConfig(dt=10.0, components=la.ALL_COMPONENTS, precision=precision, cutoff_km=10.0,
+                           threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0)
+    with la.Engine(ecfg) as eng:
+        eng.set_objects(pop)
+        want = eng.conjunction_digest(10.0)
+    d, keep = _device_objects(pop, 0, pop.n)
+    got = _noalloc(d, ecfg.to_c(), 10.0)
+    torch.cuda.synchronize()
+    print(f"dense plane: {got[0]} conjunctions (context path {want[0]})")
+    assert want[0] > 100000
+    assert got == want
