@@ -213,15 +213,16 @@ def cpu_baseline(pop, steps):
 
 
 def adjoint_leg(la, reps=3):
-    """Differentiable propagation: 4,096 objects, full force model in FP64, dt = 60 s, 1,440
-    steps (one day), checkpoints every 64 steps; loss = sum |r_T - target|^2 with targets =
-    nominal final state + N(0, 100 m).  Reports gradient-steps/s (objects x steps per
-    forward + backward pass, device time from events on torch's stream)."""
+    """Differentiable propagation: 4,096 objects of the 16,384-object workload's distributions,
+    full force model in FP64, dt = 60 s, 1,440 steps (one day), checkpoints every 64 steps;
+    loss = sum |r_T - target|^2 with targets = nominal final state + N(0, 100 m).  Reports
+    gradient-steps/s (objects x steps per forward + backward pass, device time from events on
+    torch's stream)."""
     import torch
     from ladds_amd import adjoint
 
     n, steps, dt = 4096, 1440, 60.0
-    pop = la.generate_population(1, 0x1ADD5 + 4, n)
+    pop = la.generate_population(0, 0x1ADD5 + 4, n)
     dev = torch.device("cuda:0")
     t = lambda x: torch.tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
     plan = adjoint.AdjointPlan(n, steps, dt, components=la.ALL_COMPONENTS, checkpoint_every=64)
