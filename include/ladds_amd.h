@@ -285,11 +285,13 @@ typedef struct ladds_tuning {
   int32_t workgroups_per_cu; /* pair-kernel workgroups per CU its LDS is sized for */
   double plane_width_km; /* chosen cell length along x (plane width) [km]       */
   int32_t plane_order;   /* pair-kernel dispatch: 0 plane order, 1 heaviest planes first */
+  int32_t ensemble_plane_objects; /* ensembles: objects per x-plane (0: the default 64)   */
 } ladds_tuning;
 
 LADDS_API int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuning *out);
 /* Apply a shape that ladds_autotune chose earlier (cell_size_km, plane_width_km,
- * plane_bits, pair_threads, tile_size, chunks, workgroups_per_cu, plane_order; the timing fields are
+ * plane_bits, pair_threads, tile_size, chunks, workgroups_per_cu, plane_order,
+ * ensemble_plane_objects; the timing fields are
  * ignored) without timing anything -- the counterpart of running LADDS with the
  * calibrated AutoPas configuration that tuning mode writes out (Simulation.cpp:404-407,
  * dumpCalibratedConfig).  A value outside the tuner's range is LADDS_E_INVALID. */

@@ -110,6 +110,7 @@ class Tuning(C.Structure):
         ("workgroups_per_cu", C.c_int32),
         ("plane_width_km", C.c_double),
         ("plane_order", C.c_int32),
+        ("ensemble_plane_objects", C.c_int32),
     ]
 
 

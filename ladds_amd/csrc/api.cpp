@@ -144,6 +144,7 @@ struct ladds_ctx {
   double tune_fx = 0.0;  // plane width / cell size
   int tune_wgs = 0;      // pair-kernel workgroups per CU its LDS is sized for (0: automatic)
   int tune_order = 0;    // pair-kernel dispatch: 0 plane order, 1 heaviest runs of planes first
+  int tune_ens_plane = 0;  // ensembles: objects per x-plane the planes are sized for (0: 64)
 
   StepState *d_step = nullptr;  // graph replay position (ladds_config.step_graph)
 
@@ -315,10 +316,13 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
     }
     const double nx = std::min(1.0e6, 2.0 * xmax / cell_x + 3.0);
     g.bx = std::min<uint32_t>(14u, std::max<uint32_t>(2u, ceil_log2((uint64_t)nx)));
-    // a scenario of an ensemble is a population of its own: its planes hold ~64 objects
+    // a scenario of an ensemble is a population of its own: its planes hold ~64 objects (or
+    // the tuner's choice), folding distant x-slabs into one plane so that the per-plane
+    // set-up of the pair kernel is spread over enough objects
     const size_t nscen = ctx->scen_off.size() > 2 ? ctx->scen_off.size() - 1 : 1;
     const size_t per_pop = std::max<size_t>(n / nscen, 1);
-    if (nscen > 1) g.bx = std::min<uint32_t>(g.bx, std::max<uint32_t>(2u, ceil_log2(per_pop / 64 + 1)));
+    const size_t ens_plane = ctx->tune_ens_plane > 0 ? (size_t)ctx->tune_ens_plane : 64;
+    if (nscen > 1) g.bx = std::min<uint32_t>(g.bx, std::max<uint32_t>(2u, ceil_log2(per_pop / ens_plane + 1)));
     const uint64_t used = std::min<uint64_t>((uint64_t)nx, 1ull << g.bx);
     uint32_t pb = std::min<uint32_t>(12u, std::max<uint32_t>(4u, ceil_log2(per_pop / used + 1)));
     if (ctx->tune_pb >= 4 && ctx->tune_pb <= 12) pb = (uint32_t)ctx->tune_pb;
@@ -1307,14 +1311,15 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   ctx->tune_fx = 0.0;
   ctx->tune_wgs = 0;
   ctx->tune_order = 0;
+  ctx->tune_ens_plane = 0;
   int rc = setup_grid(ctx, n, rx.data());
   if (rc != LADDS_OK) return rc;
   const double base_cell = 1.0 / ctx->grid.inv_cell;
-  const int base_pb = (int)(ctx->grid.by + ctx->grid.bz);
-  // current point of the search: {cell, plane bits, threads, tile, chunks, workgroups per CU,
-  // dispatch order}
+  const bool ensemble = ctx->tiles.nscen > 1;
+  // current point of the search: {plane bits (-1: sized from the objects per plane), threads,
+  // tile, chunks, workgroups per CU, dispatch order, ensemble objects per plane (0: 64)}
   double cur_cell = base_cell, cur_fx = 1.0;
-  int cur[6] = {base_pb, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk, 0, 0};
+  int cur[7] = {-1, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk, 0, 0, 0};
   double default_us = 0.0;
   rc = time_detection(ctx, dt_interp, repetitions, &default_us);
   if (rc != LADDS_OK) return rc;
@@ -1329,24 +1334,30 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     ctx->tune_chunks = v[3];
     ctx->tune_wgs = v[4];
     ctx->tune_order = v[5];
+    ctx->tune_ens_plane = v[6];
     return setup_grid(ctx, n, rx.data());
   };
-  // Coordinate descent, one dimension at a time, keeping every improvement: chunks per
-  // plane (load balance of dense planes), workgroup size, cell size, plane shape, tile,
-  // LDS share (workgroups per CU), dispatch order.
+  // Coordinate descent, one dimension at a time, keeping every improvement: objects per
+  // plane (ensembles), chunks per plane (load balance of dense planes), workgroup size, cell
+  // size, plane shape, buckets per plane, tile, LDS share (workgroups per CU), dispatch
+  // order.  Ensembles take a second pass: their plane size, workgroup size and buckets
+  // interact (measured on 256 scenarios: 256-object planes need 256 threads and 512 buckets).
   const int chunk_opts[] = {1, 2, 4, 8, 16, 32};
   const int thread_opts[] = {128, 256, 384, 512, 1024};
   const double cell_opts[] = {1.0, 1.25, 1.5, 2.0};
   const int tile_opts[] = {4096, 8192, 16384};
   const double fx_opts[] = {1.0, 1.5, 2.0, 3.0};
   const int wgs_opts[] = {0, 2, 3, 4, 5};
+  const int ens_opts[] = {32, 64, 128, 256, 512};
   struct Cand {
     double cell, fx;
-    std::array<int, 6> v;
+    std::array<int, 7> v;
   };
-  for (int dim = 0; dim < 8; ++dim) {
+  for (int step = 0; step < (ensemble ? 18 : 9); ++step) {
+    const int dim = step % 9;
+    if (dim == 8 && !ensemble) continue;
     std::vector<Cand> cands;
-    std::array<int, 6> c{cur[0], cur[1], cur[2], cur[3], cur[4], cur[5]};
+    std::array<int, 7> c{cur[0], cur[1], cur[2], cur[3], cur[4], cur[5], cur[6]};
     if (dim == 0)
       for (int k : chunk_opts) { c[3] = k; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 1)
@@ -1355,15 +1366,22 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
       for (double f : cell_opts) cands.push_back({base_cell * f, cur_fx, c});
     if (dim == 3)
       for (double f : fx_opts) cands.push_back({cur_cell, f, c});
-    if (dim == 4)
+    if (dim == 4) {
+      // buckets per plane around the count the current plane size gives
+      rc = apply(cur_cell, cur_fx, cur);
+      if (rc != LADDS_OK) return rc;
+      const int pb_now = (int)(ctx->grid.by + ctx->grid.bz);
       for (int d = -1; d <= 2; ++d)
-        if (base_pb + d >= 4 && base_pb + d <= 12) { c[0] = base_pb + d; cands.push_back({cur_cell, cur_fx, c}); }
+        if (pb_now + d >= 4 && pb_now + d <= 12) { c[0] = pb_now + d; cands.push_back({cur_cell, cur_fx, c}); }
+    }
     if (dim == 5)
       for (int t : tile_opts) { c[2] = t; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 6)
       for (int w : wgs_opts) { c[4] = w; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 7)
       for (int o = 0; o < 2; ++o) { c[5] = o; cands.push_back({cur_cell, cur_fx, c}); }
+    if (dim == 8)  // ensembles: objects per plane, buckets sized from it again
+      for (int e : ens_opts) { c[6] = e; c[0] = -1; cands.push_back({cur_cell, cur_fx, c}); }
     for (const Cand &cand : cands) {
       if (cand.cell == cur_cell && cand.fx == cur_fx && std::equal(cand.v.begin(), cand.v.end(), cur)) continue;
       rc = apply(cand.cell, cand.fx, cand.v.data());
@@ -1376,7 +1394,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
         best_us = us;
         cur_cell = cand.cell;
         cur_fx = cand.fx;
-        for (int k = 0; k < 6; ++k) cur[k] = cand.v[k];
+        for (int k = 0; k < 7; ++k) cur[k] = cand.v[k];
       }
     }
   }
@@ -1394,6 +1412,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     out->chunks = (int32_t)ctx->tiles.nchunk;
     out->workgroups_per_cu = (int32_t)ctx->tune_wgs;
     out->plane_order = (int32_t)ctx->tune_order;
+    out->ensemble_plane_objects = (int32_t)ctx->tune_ens_plane;
     out->scan_us = best_us;
     out->default_us = default_us;
     out->tried = tried;
@@ -1429,7 +1448,8 @@ int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *t) {
                   valid_pair_threads(t->pair_threads) &&
                   (t->tile_size == 4096 || t->tile_size == 8192 || t->tile_size == 16384) && t->chunks >= 1 &&
                   t->chunks <= 32 && t->workgroups_per_cu >= 0 && t->workgroups_per_cu <= 8 &&
-                  (t->plane_order == 0 || t->plane_order == 1);
+                  (t->plane_order == 0 || t->plane_order == 1) && t->ensemble_plane_objects >= 0 &&
+                  t->ensemble_plane_objects <= 65536;
   if (!ok) return ctx->fail(LADDS_E_INVALID, "tuning values outside the tuner's range");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   ctx->tune_cell_km = t->cell_size_km;
@@ -1440,6 +1460,7 @@ int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *t) {
   ctx->tune_chunks = t->chunks;
   ctx->tune_wgs = t->workgroups_per_cu;
   ctx->tune_order = t->plane_order;
+  ctx->tune_ens_plane = t->ensemble_plane_objects;
   if (ctx->n == 0) return LADDS_OK;  // applied at the next ladds_set_objects
   std::vector<double> rx(ctx->n);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -156,3 +156,30 @@ def test_graph_replay_matches_eager_launches(ensemble):
         assert np.array_equal(a[4], b[4]) and np.array_equal(a[5], b[5])
         assert a[6] == b[6]
     assert sum(x[0] for x in eager) > 0
+
+
+@pytest.mark.parametrize("plane_objects", [32, 256, 512])
+def test_ensemble_plane_sizes_keep_pairs(plane_objects):
+    """Ensemble planes sized for more (or fewer) objects fold more (or fewer) distant x-slabs
+    into one plane of a scenario; each scenario's pair set still equals the oracle's."""
+    pop, off = _ensemble()
+    cfg = dict(dt=10.0, components=la.KEP | la.J2, precision=la.SPLIT, cutoff_km=10.0,
+               threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0)
+    with la.Engine(la.EngineConfig(**cfg)) as eng:
+        eng.set_objects(pop)
+        eng.set_scenarios(off)
+        t = eng.autotune(repetitions=1)
+        eng.set_tuning(dict(t, ensemble_plane_objects=plane_objects))
+        eng.propagate(2)
+        hits, ev = eng.conjunctions()
+        r, v = eng.state()
+    ocfg = ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0)
+    total = 0
+    for s in range(len(SIZES)):
+        oh, oe = _scenario_oracle(pop, off, s, r, v, ocfg)
+        mine = [h for h in hits if (int(h["id_a"]) >> 32) == s]
+        mine_e = [h for h in ev if (int(h["id_a"]) >> 32) == s]
+        assert _key(mine) == _key(oh, int(off[s])), s
+        assert _key(mine_e) == _key(oe, int(off[s])), s
+        total += len(mine)
+    assert total >= 8 * 24 // 2
