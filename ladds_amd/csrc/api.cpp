@@ -401,25 +401,29 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   }
   tp.plane_bits = g.by + g.bz;
   tp.nplanes = 1u << g.bx;
-  if (tiles && tp.nscen == 1 && !plane_count.empty() && ctx->tune_order == 1) {
-    // Dispatch order of the pair kernel's plane pairs: workgroups start in index order and
-    // each XCD takes runs of 8 consecutive plane pairs, so runs of 8 planes are put in
-    // descending order of their work (longest first: the last workgroups on a CU are the
-    // short ones).  Work of a plane pair ~ 6 x own records (scan) + next records (gather).
-    // Measured at 1M objects: -6% for FP64 records with 512 threads, +7% for split records
-    // with 256 threads, so the tuner decides.
+  if (tiles && !plane_count.empty() && plane_count.size() == (size_t)tp.nplanes * tp.nscen &&
+      ctx->tune_order == 1) {
+    // Dispatch order of the pair kernel's plane pairs (scenario x plane): workgroups start
+    // in index order and each XCD takes runs of 8 consecutive plane pairs, so runs of 8
+    // planes of one scenario are put in descending order of their work (longest first: the
+    // last workgroups on a CU are the short ones).  Work of a plane pair ~ 6 x own records
+    // (scan) + next records (gather).  Measured at 1M objects: -6% for FP64 records with
+    // 512 threads, +7% for split records with 256 threads, so the tuner decides.
     const uint32_t P = tp.nplanes, run = 8;
     std::vector<std::pair<uint64_t, uint32_t>> w;
-    for (uint32_t r0 = 0; r0 < P; r0 += run) {
-      uint64_t s = 0;
-      for (uint32_t p = r0; p < r0 + run && p < P; ++p) s += 6ull * plane_count[p] + plane_count[(p + 1) % P];
-      w.emplace_back(s, r0);
+    for (uint32_t sc = 0; sc < tp.nscen; ++sc) {
+      const uint32_t *pc = plane_count.data() + (size_t)sc * P;
+      for (uint32_t r0 = 0; r0 < P; r0 += run) {
+        uint64_t s = 0;
+        for (uint32_t p = r0; p < r0 + run && p < P; ++p) s += 6ull * pc[p] + pc[(p + 1) % P];
+        w.emplace_back(s, sc * P + r0);
+      }
     }
     std::stable_sort(w.begin(), w.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
     std::vector<uint32_t> order;
-    order.reserve(P);
+    order.reserve((size_t)P * tp.nscen);
     for (const auto &e : w)
-      for (uint32_t p = e.second; p < e.second + run && p < P; ++p) order.push_back(p);
+      for (uint32_t k = 0; k < run && (e.second % P) + k < P; ++k) order.push_back(e.second + k);
     if (order.size() > ctx->plane_order_alloc) {
       hipFree(ctx->d_plane_order);
       ctx->d_plane_order = nullptr;

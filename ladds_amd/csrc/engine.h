@@ -69,7 +69,8 @@ struct TileParams {
   // kernel's arena planes read the predicate's fields from them (set when plane pairs
   // exceeded the LDS capacity at set-up: dense, clustered populations)
   uint32_t arena_fat;
-  // One population: the planes in dispatch order (nplanes entries), or null for plane order.
+  // The plane pairs (scenario * nplanes + plane) in dispatch order (nscen * nplanes
+  // entries), or null for index order.
   const uint32_t *plane_order;
 };
 
