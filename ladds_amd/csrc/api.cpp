@@ -171,6 +171,11 @@ struct ladds_ctx {
   unsigned long long *xcount = nullptr;
 
   uint64_t step_count = 0;  // steps since t0
+  // Environment segment staged by ladds_propagate: steps [env_first, env_first + env_len)
+  // of the clock (env_t0, env_dt) are in env64/env32 from index 0 (reused by later calls;
+  // ladds_simulate stages its own and clears env_len).
+  double env_t0 = 0.0, env_dt = 0.0;
+  uint64_t env_first = 0, env_len = 0;
   double c[4], dcoef[3];
 
   bool profiling = false;
@@ -1244,10 +1249,23 @@ int ladds_propagate(ladds_ctx *ctx, int nsteps) {
   const bool tiles = ctx->variant == kVariantPlaneTiles;
   for (int seg0 = 0; seg0 < nsteps; seg0 += kEnvSegment) {
     const int seg = std::min(kEnvSegment, nsteps - seg0);
-    int rc = stage_env(ctx, 0, seg);
+    // The environment of a whole segment is staged once and serves the calls that follow
+    // (integrate(false) once per iteration) until the clock leaves it.
+    const bool cached = ctx->env_len && ctx->env_t0 == ctx->cfg.t0 && ctx->env_dt == ctx->cfg.dt &&
+                        ctx->step_count >= ctx->env_first &&
+                        ctx->step_count + (uint64_t)seg <= ctx->env_first + ctx->env_len;
+    int rc = LADDS_OK;
+    if (!cached) {
+      rc = stage_env(ctx, 0, kEnvSegment);
+      ctx->env_t0 = ctx->cfg.t0;
+      ctx->env_dt = ctx->cfg.dt;
+      ctx->env_first = ctx->step_count;
+      ctx->env_len = rc == LADDS_OK ? (uint64_t)kEnvSegment : 0;
+    }
+    const size_t off = (size_t)(ctx->step_count - ctx->env_first);
     for (int s = 0; s < seg && rc == LADDS_OK; s += kChunk) {
       const int k = std::min(kChunk, seg - s);
-      rc = launch_propagate_steps(ctx, (size_t)s, k, tiles && seg0 + s + k == nsteps, 0.0);
+      rc = launch_propagate_steps(ctx, off + (size_t)s, k, tiles && seg0 + s + k == nsteps, 0.0);
     }
     if (rc != LADDS_OK) return rc;
     ctx->step_count += (uint64_t)seg;
@@ -1492,7 +1510,9 @@ int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_
       HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&ctx->h_log), sizeof(LoggedPair) * 4096, 0));
       ctx->h_log_cap = 4096;
     }
-    const size_t staged = std::min<size_t>(ctx->h_log_cap, ctx->log_cap);
+    // the first 256 entries come back with the counters (a call that finds more copies the
+    // rest after the first synchronisation)
+    const size_t staged = std::min<size_t>({ctx->h_log_cap, ctx->log_cap, 256});
     HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->sc, ctx->counters, sizeof(ScanCounters), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->log_count, ctx->log_count, sizeof(unsigned long long),
                                hipMemcpyDeviceToHost, ctx->stream));
@@ -1506,15 +1526,17 @@ int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_
       if (rc != LADDS_OK) return rc;
       continue;
     }
-    if (entries > staged) {  // more than the staging holds: grow it, copy the rest
-      size_t want = ctx->h_log_cap;
-      while (want < entries) want *= 2;
-      LoggedPair *bigger = nullptr;
-      HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&bigger), sizeof(LoggedPair) * want, 0));
-      std::memcpy(bigger, ctx->h_log, sizeof(LoggedPair) * staged);
-      hipHostFree(ctx->h_log);
-      ctx->h_log = bigger;
-      ctx->h_log_cap = want;
+    if (entries > staged) {  // more than staged: grow the pinned block if needed, copy the rest
+      if (entries > ctx->h_log_cap) {
+        size_t want = ctx->h_log_cap;
+        while (want < entries) want *= 2;
+        LoggedPair *bigger = nullptr;
+        HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&bigger), sizeof(LoggedPair) * want, 0));
+        std::memcpy(bigger, ctx->h_log, sizeof(LoggedPair) * staged);
+        hipHostFree(ctx->h_log);
+        ctx->h_log = bigger;
+        ctx->h_log_cap = want;
+      }
       HIPCHK(ctx, hipMemcpyAsync(ctx->h_log + staged, ctx->log + staged, sizeof(LoggedPair) * (entries - staged),
                                  hipMemcpyDeviceToHost, ctx->stream));
       HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1651,7 +1673,9 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   for (int seg0 = 0; seg0 < nsteps && ctx->n > 0; seg0 += kEnvSegment) {
     const int seg = std::min(kEnvSegment, nsteps - seg0);
     // environments of the whole segment, in stream order after the previous segment's steps
+    // (they replace the segment ladds_propagate staged)
     {
+      ctx->env_len = 0;
       const int rc = stage_env(ctx, 0, seg);
       if (rc != LADDS_OK) {
         drop_graphs();
