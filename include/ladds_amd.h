@@ -207,9 +207,12 @@ LADDS_API int ladds_get_ids(ladds_ctx *ctx, uint64_t *id);
 LADDS_API double ladds_get_time(const ladds_ctx *ctx);
 LADDS_API int ladds_set_time(ladds_ctx *ctx, double t);
 
-/* YoshidaIntegrator::integrate(false), nsteps times (Simulation.cpp:297).  Runs the steps
- * on the context's stream and returns when they are done.  The Sun/Moon/Earth-rotation
- * environment of 8,192 steps is staged at once and reused by the calls that follow. */
+/* YoshidaIntegrator::integrate(false), nsteps times (Simulation.cpp:297).  Enqueues the
+ * steps on the context's stream and returns without waiting: a detection that follows
+ * (ladds_conjunctions, ladds_conjunction_digest) is ordered after them on that stream, and
+ * every call that reads or writes object data from the host waits for them first (as does
+ * ladds_synchronize).  The Sun/Moon/Earth-rotation environment of 8,192 steps is staged at
+ * once and reused by the calls that follow. */
 LADDS_API int ladds_propagate(ladds_ctx *ctx, int nsteps);
 
 /* Simulation::collisionDetection (Simulation.cpp:211-222): AutoPas::iteratePairwise with a

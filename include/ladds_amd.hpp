@@ -29,6 +29,7 @@
 #include <charconv>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <fstream>
 #include <limits>
 #include <sstream>
@@ -238,12 +239,17 @@ class Container {
   ladds_ctx *context() { return ctx_; }
   ladds_config &config() { return cfg_; }
   void applyConfig() {
+    // unchanged since the last call (iteratePairwise every iteration with the same functor
+    // parameters): nothing to send
+    if (configured_ && std::memcmp(&cfg_, &applied_cfg_, sizeof(cfg_)) == 0) return;
     const bool dt_only = cfg_.dt != applied_dt_ && cfg_.t0 == applied_t0_;
     check(ladds_configure(ctx_, &cfg_), ctx_, "ladds_configure");
     // a new step alone keeps the elapsed time: the context's clock restarts at the current time
     if (dt_only) cfg_.t0 = ladds_get_time(ctx_);
     applied_dt_ = cfg_.dt;
     applied_t0_ = cfg_.t0;
+    applied_cfg_ = cfg_;
+    configured_ = true;
   }
   /* Upload the host mirror if it was changed since the last upload: the whole mirror after
    * host-side edits, only the new tail after addParticle calls (breakup fragments). */
@@ -312,7 +318,11 @@ class Container {
   bool host_stale_ = false;
   size_t appended_ = 0;
   double applied_dt_ = -1.0, applied_t0_ = 0.0;  // the configuration the context holds
+  ladds_config applied_cfg_{};
+  bool configured_ = false;
   bool tuned_ = false;
+  // pair buffers reused by every iteratePairwise (grown when a detection reports more)
+  std::vector<ladds_pair> hit_buf_, ev_buf_;
 
   void upload(size_t b, size_t e, int (*entry)(ladds_ctx *, const ladds_objects *), const char *what) {
     const size_t n = e - b;
@@ -421,38 +431,42 @@ inline bool Container::iteratePairwise(CollisionFunctor *f) {
     tuned_ = tuning = true;
   }
   size_t n = 0, ne = 0;
-  std::vector<ladds_pair> hits(4096), evaded(4096);
-  int rc = ladds_conjunctions(ctx_, f->dt_, hits.data(), hits.size(), &n, evaded.data(), evaded.size(), &ne);
+  if (hit_buf_.size() < 4096) hit_buf_.resize(4096);
+  if (ev_buf_.size() < 4096) ev_buf_.resize(4096);
+  int rc = ladds_conjunctions(ctx_, f->dt_, hit_buf_.data(), hit_buf_.size(), &n, ev_buf_.data(), ev_buf_.size(), &ne);
   if (rc == LADDS_E_OVERFLOW) {  // rerun with the sizes the first pass reported
-    hits.resize(n);
-    evaded.resize(ne);
-    rc = ladds_conjunctions(ctx_, f->dt_, hits.data(), hits.size(), &n, evaded.data(), evaded.size(), &ne);
+    hit_buf_.resize(std::max(n, hit_buf_.size()));
+    ev_buf_.resize(std::max(ne, ev_buf_.size()));
+    rc = ladds_conjunctions(ctx_, f->dt_, hit_buf_.data(), hit_buf_.size(), &n, ev_buf_.data(), ev_buf_.size(), &ne);
   }
   check(rc, ctx_, "ladds_conjunctions");
-  hits.resize(n);
-  evaded.resize(ne);
+  const ladds_pair *hits = hit_buf_.data(), *evaded = ev_buf_.data();
   // only the objects the tuples point at are brought up to date (the rest of the mirror
   // stays behind the device until it is read)
   std::vector<uint32_t> touched;
   touched.reserve(2 * (n + ne));
-  for (const auto *list : {&hits, &evaded})
-    for (const ladds_pair &p : *list) {
-      touched.push_back(p.idx_a);
-      touched.push_back(p.idx_b);
-    }
+  for (size_t k = 0; k < n; ++k) {
+    touched.push_back(hits[k].idx_a);
+    touched.push_back(hits[k].idx_b);
+  }
+  for (size_t k = 0; k < ne; ++k) {
+    touched.push_back(evaded[k].idx_a);
+    touched.push_back(evaded[k].idx_b);
+  }
   std::sort(touched.begin(), touched.end());
   touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
   refresh(touched);
-  auto convert = [this](const std::vector<ladds_pair> &src, CollisionFunctor::CollisionCollectionT &dst) {
+  auto convert = [this](const ladds_pair *src, size_t count, CollisionFunctor::CollisionCollectionT &dst) {
     dst.clear();
-    dst.reserve(src.size());
-    for (const ladds_pair &p : src) {
+    dst.reserve(count);
+    for (size_t k = 0; k < count; ++k) {
+      const ladds_pair &p = src[k];
       Particle *a = particleAt(p.idx_a), *b = particleAt(p.idx_b);
       dst.emplace_back(a, b, p.d2, std::array<double, 3>{p.mid[0], p.mid[1], p.mid[2]});
     }
   };
-  convert(hits, f->collisions_);
-  convert(evaded, f->evaded_);
+  convert(hits, n, f->collisions_);
+  convert(evaded, ne, f->evaded_);
   return tuning;
 }
 

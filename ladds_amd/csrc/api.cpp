@@ -176,6 +176,9 @@ struct ladds_ctx {
   // ladds_simulate stages its own and clears env_len).
   double env_t0 = 0.0, env_dt = 0.0;
   uint64_t env_first = 0, env_len = 0;
+  // ladds_propagate returned with its steps still running on `stream`; entry points that
+  // touch device memory from the host first wait for them (settle)
+  bool pending = false;
   double c[4], dcoef[3];
 
   bool profiling = false;
@@ -547,6 +550,20 @@ int stage_env(ladds_ctx *ctx, int first, int count) {
                                  ctx->env32, ctx->stream));
   return LADDS_OK;
 }
+
+// Waits for steps ladds_propagate left running (see ladds_ctx::pending).
+int settle(ladds_ctx *ctx) {
+  if (!ctx->pending) return LADDS_OK;
+  ctx->pending = false;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return LADDS_OK;
+}
+#define LADDS_SETTLE(ctx)                  \
+  do {                                     \
+    const int settle_rc = settle(ctx);     \
+    if (settle_rc != LADDS_OK) return settle_rc; \
+  } while (0)
 
 int ensure_log(ladds_ctx *ctx, uint64_t cap) {
   if (cap <= ctx->log_cap) return LADDS_OK;
@@ -968,6 +985,7 @@ const char *ladds_last_error(const ladds_ctx *ctx) {
 
 int ladds_configure(ladds_ctx *ctx, const ladds_config *cfg) {
   if (!ctx || !cfg) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   if (!(cfg->dt > 0.0) || cfg->threshold_mode < 0 || cfg->threshold_mode > 1 || cfg->pair_output < 0 ||
       cfg->pair_output > 1)
     return ctx->fail(LADDS_E_INVALID, "invalid dt, threshold_mode or pair_output");
@@ -1129,6 +1147,7 @@ extern "C" {
 
 int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
   if (!ctx || !o) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   if (o->n >= (size_t)UINT32_MAX) return ctx->fail(LADDS_E_INVALID, "at most 2^32-2 objects per context");
   int rc = check_objects(ctx, o);
   if (rc != LADDS_OK) return rc;
@@ -1148,6 +1167,7 @@ int ladds_set_objects(ladds_ctx *ctx, const ladds_objects *o) {
 
 int ladds_add_objects(ladds_ctx *ctx, const ladds_objects *o) {
   if (!ctx || !o) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   if (ctx->scen_off.size() > 2) return ctx->fail(LADDS_E_STATE, "objects cannot be added to an ensemble");
   if (ctx->n + o->n >= (size_t)UINT32_MAX) return ctx->fail(LADDS_E_INVALID, "at most 2^32-2 objects per context");
   int rc = check_objects(ctx, o);
@@ -1169,6 +1189,7 @@ int ladds_add_objects(ladds_ctx *ctx, const ladds_objects *o) {
 
 int ladds_set_scenarios(ladds_ctx *ctx, size_t nscenarios, const uint64_t *offsets) {
   if (!ctx || nscenarios == 0 || !offsets) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   if (ctx->n == 0) return ctx->fail(LADDS_E_STATE, "set objects before scenarios");
   if (ctx->variant != kVariantPlaneTiles) return ctx->fail(LADDS_E_STATE, "scenarios need the plane-tile scan");
   if (offsets[0] != 0 || offsets[nscenarios] != ctx->n)
@@ -1206,6 +1227,7 @@ size_t ladds_num_objects(const ladds_ctx *ctx) { return ctx ? ctx->n : 0; }
 
 int ladds_get_state(ladds_ctx *ctx, double *rx, double *ry, double *rz, double *vx, double *vy, double *vz) {
   if (!ctx) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const size_t b = 8 * ctx->n;
   double *dst[6] = {rx, ry, rz, vx, vy, vz};
@@ -1218,6 +1240,7 @@ int ladds_get_state(ladds_ctx *ctx, double *rx, double *ry, double *rz, double *
 
 int ladds_get_ids(ladds_ctx *ctx, uint64_t *id) {
   if (!ctx || !id) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   if (ctx->n) HIPCHK(ctx, hipMemcpyAsync(id, ctx->d.id, 8 * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1271,7 +1294,9 @@ int ladds_propagate(ladds_ctx *ctx, int nsteps) {
     ctx->step_count += (uint64_t)seg;
   }
   ctx->keys_valid = tiles;
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  // no wait here: a detection that follows is ordered on the same stream, and calls that
+  // touch device memory from the host settle first
+  ctx->pending = true;
   return LADDS_OK;
 }
 
@@ -1285,6 +1310,7 @@ static int run_detection_sync(ladds_ctx *ctx, double dt_interp, bool digest, Sca
   HIPCHK(ctx, hipMemcpyAsync(out_counters, ctx->counters, sizeof(ScanCounters), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->log_count, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->pending = false;
   *entries = cnt;
   if (out_counters->overflow) return ctx->fail(LADDS_E_OVERFLOW, "conjunction scan scratch arena exhausted");
   return LADDS_OK;
@@ -1315,6 +1341,7 @@ static int time_detection(ladds_ctx *ctx, double dt_interp, int reps, double *us
 
 int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuning *out) {
   if (!ctx || repetitions < 1) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   if (ctx->n < 2) return ctx->fail(LADDS_E_STATE, "set objects before autotuning");
   if (ctx->variant != kVariantPlaneTiles) return ctx->fail(LADDS_E_STATE, "autotuning covers the plane-tile scan");
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1463,6 +1490,7 @@ extern "C" {
 
 int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *t) {
   if (!ctx || !t) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   if (ctx->variant != kVariantPlaneTiles) return ctx->fail(LADDS_E_STATE, "tuning covers the plane-tile scan");
   const double radius = search_radius(ctx->form, ctx->cfg.cutoff_km);
   const bool ok = t->cell_size_km >= radius * (1.0 + 1e-9) && t->plane_width_km >= t->cell_size_km &&
@@ -1519,6 +1547,7 @@ int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_
     HIPCHK(ctx, hipMemcpyAsync(ctx->h_log, ctx->log, sizeof(LoggedPair) * staged, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->pending = false;
     if (ctx->h_rb->sc.overflow) return ctx->fail(LADDS_E_OVERFLOW, "conjunction scan scratch arena exhausted");
     entries = ctx->h_rb->log_count;
     if (entries > ctx->log_cap) {  // the log was too small: grow it and scan again
@@ -1562,6 +1591,7 @@ int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_
 
 int ladds_gather_state(ladds_ctx *ctx, const uint32_t *idx, size_t count, double *r, double *v) {
   if (!ctx || (count && (!idx || !r || !v))) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   if (count == 0) return LADDS_OK;
   for (size_t k = 0; k < count; ++k)
     if (idx[k] >= ctx->n) return ctx->fail(LADDS_E_INVALID, "object index out of range");
@@ -1622,6 +1652,7 @@ int ladds_conjunction_digest(ladds_ctx *ctx, double dt_interp, uint64_t *n, uint
 int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpcd, double min_altitude_km,
                    ladds_run_stats *stats) {
   if (!ctx || nsteps < 0 || tpcd < 1) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   if (stats) std::memset(stats, 0, sizeof(*stats));
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
@@ -1800,6 +1831,7 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
 
 int ladds_remove_spawn_protection(ladds_ctx *ctx) {
   if (!ctx) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, launch_clear_parents(nullptr, 1, ctx->d.parent, ctx->n, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1809,6 +1841,7 @@ int ladds_remove_spawn_protection(ladds_ctx *ctx) {
 
 int ladds_fetch_log(ladds_ctx *ctx, ladds_logged_pair *out, size_t cap, size_t *n) {
   if (!ctx || !n || (cap && !out)) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   unsigned long long cnt = 0;
   HIPCHK(ctx, hipMemcpyAsync(&cnt, ctx->log_count, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
@@ -1830,6 +1863,7 @@ extern "C" {
 
 int ladds_particle_records(ladds_ctx *ctx, float *pos, float *vel, uint32_t *ids) {
   if (!ctx) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   const size_t n = ctx->n;
   if (n && (!pos || !vel || !ids)) return LADDS_E_INVALID;
   if (n == 0) return LADDS_OK;
@@ -1851,6 +1885,7 @@ int ladds_particle_records(ladds_ctx *ctx, float *pos, float *vel, uint32_t *ids
 int ladds_log_records(ladds_ctx *ctx, uint64_t iteration, ladds_pair_record *hits, size_t cap, size_t *n,
                       ladds_pair_record *evaded, size_t ecap, size_t *ne) {
   if (!ctx || !n || !ne || (cap && !hits) || (ecap && !evaded)) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const size_t rb = sizeof(PairRecord32);
   int rc = ensure_xbuf(ctx, std::max<size_t>(rb * (cap + ecap), 64));
@@ -1876,6 +1911,7 @@ int ladds_log_records(ladds_ctx *ctx, uint64_t iteration, ladds_pair_record *hit
 
 int ladds_clear_log(ladds_ctx *ctx) {
   if (!ctx) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1885,6 +1921,7 @@ int ladds_clear_log(ladds_ctx *ctx) {
 
 int ladds_delete_burnups(ladds_ctx *ctx, double min_altitude_km, size_t *removed) {
   if (!ctx) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const size_t before = ctx->n;
   if (before) {
@@ -1915,6 +1952,7 @@ int ladds_delete_burnups(ladds_ctx *ctx, double min_altitude_km, size_t *removed
 
 int ladds_delete_objects(ladds_ctx *ctx, const uint64_t *ids, size_t count, size_t *removed) {
   if (!ctx || (count && !ids)) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const size_t before = ctx->n;
   if (before && count) {
@@ -1942,6 +1980,7 @@ int ladds_delete_objects(ladds_ctx *ctx, const uint64_t *ids, size_t count, size
 
 int ladds_set_profiling(ladds_ctx *ctx, int enable) {
   if (!ctx) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   resolve_profiles(ctx);
   ctx->profiling = enable != 0;
   for (auto &kv : ctx->prof) {
@@ -1953,6 +1992,7 @@ int ladds_set_profiling(ladds_ctx *ctx, int enable) {
 
 int ladds_kernel_stats(ladds_ctx *ctx, const char *name, uint64_t *launches, double *total_ms) {
   if (!ctx || !name || !launches || !total_ms) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   bool known = false;
   for (const char *k : kKernelNames) known |= std::strcmp(k, name) == 0;
   if (!known) return ctx->fail(LADDS_E_INVALID, std::string("unknown kernel name ") + name);
@@ -1965,6 +2005,7 @@ int ladds_kernel_stats(ladds_ctx *ctx, const char *name, uint64_t *launches, dou
 
 int ladds_synchronize(ladds_ctx *ctx) {
   if (!ctx) return LADDS_E_INVALID;
+  LADDS_SETTLE(ctx);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return LADDS_OK;
 }
