@@ -289,12 +289,17 @@ typedef struct ladds_tuning {
   double plane_width_km; /* chosen cell length along x (plane width) [km]       */
   int32_t plane_order;   /* pair-kernel dispatch: 0 plane order, 1 heaviest planes first */
   int32_t ensemble_plane_objects; /* ensembles: objects per x-plane (0: the default 64)   */
+  int32_t row_cells;     /* bucket row height along y in cells: 1 (an object scans three */
+                         /* rows of the next plane) or 2 (two rows, on the side of the   */
+                         /* half of its row it lies in); 0 is read as 1                  */
+  int32_t bucket_cells;  /* bucket width along z in cells: 1 (three buckets of a row) or  */
+                         /* 2 (two, on the side of the half of its bucket); 0 is read as 1 */
 } ladds_tuning;
 
 LADDS_API int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuning *out);
 /* Apply a shape that ladds_autotune chose earlier (cell_size_km, plane_width_km,
  * plane_bits, pair_threads, tile_size, chunks, workgroups_per_cu, plane_order,
- * ensemble_plane_objects; the timing fields are
+ * ensemble_plane_objects, row_cells, bucket_cells; the timing fields are
  * ignored) without timing anything -- the counterpart of running LADDS with the
  * calibrated AutoPas configuration that tuning mode writes out (Simulation.cpp:404-407,
  * dumpCalibratedConfig).  A value outside the tuner's range is LADDS_E_INVALID. */

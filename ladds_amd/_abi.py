@@ -111,6 +111,8 @@ class Tuning(C.Structure):
         ("plane_width_km", C.c_double),
         ("plane_order", C.c_int32),
         ("ensemble_plane_objects", C.c_int32),
+        ("row_cells", C.c_int32),
+        ("bucket_cells", C.c_int32),
     ]
 
 

@@ -145,6 +145,8 @@ struct ladds_ctx {
   int tune_wgs = 0;      // pair-kernel workgroups per CU its LDS is sized for (0: automatic)
   int tune_order = 0;    // pair-kernel dispatch: 0 plane order, 1 heaviest runs of planes first
   int tune_ens_plane = 0;  // ensembles: objects per x-plane the planes are sized for (0: 64)
+  int tune_rows = 0;       // bucket row height along y in cells: 2 or 1 (0: 1)
+  int tune_bcells = 0;     // bucket width along z in cells: 2 or 1 (0: 1)
 
   StepState *d_step = nullptr;  // graph replay position (ladds_config.step_graph)
 
@@ -314,6 +316,12 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   // plane width: cells may be longer in x than in y and z (fewer, fuller planes)
   const double cell_x = (tiles && ctx->tune_fx > 1.0) ? cell * ctx->tune_fx : cell;
   g.inv_cell_x = 1.0 / cell_x;
+  // buckets two cells tall (wide): an object then scans two rows of the next plane (two
+  // buckets of a row), not three
+  const bool half_rows = tiles && ctx->tune_rows == 2;
+  const bool half_buckets = tiles && ctx->tune_bcells == 2;
+  g.inv_cell_y = half_rows ? 0.5 * g.inv_cell : g.inv_cell;
+  g.inv_cell_z = half_buckets ? 0.5 * g.inv_cell : g.inv_cell;
   uint32_t bits;
   std::vector<uint32_t> plane_count;
   if (tiles) {
@@ -334,7 +342,9 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
     const uint64_t used = std::min<uint64_t>((uint64_t)nx, 1ull << g.bx);
     uint32_t pb = std::min<uint32_t>(12u, std::max<uint32_t>(4u, ceil_log2(per_pop / used + 1)));
     if (ctx->tune_pb >= 4 && ctx->tune_pb <= 12) pb = (uint32_t)ctx->tune_pb;
-    g.by = pb / 2;
+    // rows twice as tall as the buckets are wide fold y with one bit fewer, so that both
+    // folds span similar lengths
+    g.by = (half_rows && !half_buckets) ? (pb - 1) / 2 : pb / 2;
     g.bz = pb - g.by;
     bits = g.bx + pb;
     if (rx && n) {
@@ -409,6 +419,8 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
   }
   tp.plane_bits = g.by + g.bz;
   tp.nplanes = 1u << g.bx;
+  tp.half_rows = half_rows ? 1u : 0u;
+  tp.half_buckets = half_buckets ? 1u : 0u;
   if (tiles && !plane_count.empty() && plane_count.size() == (size_t)tp.nplanes * tp.nscen &&
       ctx->tune_order == 1) {
     // Dispatch order of the pair kernel's plane pairs (scenario x plane): workgroups start
@@ -1361,14 +1373,17 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   ctx->tune_wgs = 0;
   ctx->tune_order = 0;
   ctx->tune_ens_plane = 0;
+  ctx->tune_rows = 0;
+  ctx->tune_bcells = 0;
   int rc = setup_grid(ctx, n, rx.data());
   if (rc != LADDS_OK) return rc;
   const double base_cell = 1.0 / ctx->grid.inv_cell;
   const bool ensemble = ctx->tiles.nscen > 1;
   // current point of the search: {plane bits (-1: sized from the objects per plane), threads,
-  // tile, chunks, workgroups per CU, dispatch order, ensemble objects per plane (0: 64)}
+  // tile, chunks, workgroups per CU, dispatch order, ensemble objects per plane (0: 64), row
+  // height and bucket width in cells}
   double cur_cell = base_cell, cur_fx = 1.0;
-  int cur[7] = {-1, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk, 0, 0, 0};
+  int cur[9] = {-1, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk, 0, 0, 0, 1, 1};
   double default_us = 0.0;
   rc = time_detection(ctx, dt_interp, repetitions, &default_us);
   if (rc != LADDS_OK) return rc;
@@ -1384,13 +1399,16 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     ctx->tune_wgs = v[4];
     ctx->tune_order = v[5];
     ctx->tune_ens_plane = v[6];
+    ctx->tune_rows = v[7];
+    ctx->tune_bcells = v[8];
     return setup_grid(ctx, n, rx.data());
   };
-  // Coordinate descent, one dimension at a time, keeping every improvement: objects per
-  // plane (ensembles), chunks per plane (load balance of dense planes), workgroup size, cell
-  // size, plane shape, buckets per plane, tile, LDS share (workgroups per CU), dispatch
-  // order.  Ensembles take a second pass: their plane size, workgroup size and buckets
-  // interact (measured on 256 scenarios: 256-object planes need 256 threads and 512 buckets).
+  // Coordinate descent, one dimension at a time, keeping every improvement: chunks per
+  // plane (load balance of dense planes), workgroup size, bucket shape, cell size, plane
+  // shape, buckets per plane, tile, LDS share (workgroups per CU), dispatch order, objects
+  // per plane (ensembles).  Ensembles take a second pass: their plane size, workgroup size
+  // and buckets interact (measured on 256 scenarios: 256-object planes need 256 threads and
+  // 512 buckets).
   const int chunk_opts[] = {1, 2, 4, 8, 16, 32};
   const int thread_opts[] = {128, 256, 384, 512, 1024};
   const double cell_opts[] = {1.0, 1.25, 1.5, 2.0};
@@ -1400,13 +1418,16 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   const int ens_opts[] = {32, 64, 128, 256, 512};
   struct Cand {
     double cell, fx;
-    std::array<int, 7> v;
+    std::array<int, 9> v;
   };
-  for (int step = 0; step < (ensemble ? 18 : 9); ++step) {
-    const int dim = step % 9;
+  // dimensions in search order (9: bucket shape)
+  const int dims[] = {0, 1, 9, 2, 3, 4, 5, 6, 7, 8};
+  constexpr int kDims = (int)(sizeof(dims) / sizeof(dims[0]));
+  for (int step = 0; step < (ensemble ? 2 * kDims : kDims); ++step) {
+    const int dim = dims[step % kDims];
     if (dim == 8 && !ensemble) continue;
     std::vector<Cand> cands;
-    std::array<int, 7> c{cur[0], cur[1], cur[2], cur[3], cur[4], cur[5], cur[6]};
+    std::array<int, 9> c{cur[0], cur[1], cur[2], cur[3], cur[4], cur[5], cur[6], cur[7], cur[8]};
     if (dim == 0)
       for (int k : chunk_opts) { c[3] = k; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 1)
@@ -1431,6 +1452,12 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
       for (int o = 0; o < 2; ++o) { c[5] = o; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 8)  // ensembles: objects per plane, buckets sized from it again
       for (int e : ens_opts) { c[6] = e; c[0] = -1; cands.push_back({cur_cell, cur_fx, c}); }
+    if (dim == 9)  // bucket shape in cells (y x z): 1 x 1, 2 x 1, 2 x 2
+      for (int k = 0; k < 3; ++k) {
+        c[7] = k ? 2 : 1;
+        c[8] = k == 2 ? 2 : 1;
+        cands.push_back({cur_cell, cur_fx, c});
+      }
     for (const Cand &cand : cands) {
       if (cand.cell == cur_cell && cand.fx == cur_fx && std::equal(cand.v.begin(), cand.v.end(), cur)) continue;
       rc = apply(cand.cell, cand.fx, cand.v.data());
@@ -1443,7 +1470,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
         best_us = us;
         cur_cell = cand.cell;
         cur_fx = cand.fx;
-        for (int k = 0; k < 7; ++k) cur[k] = cand.v[k];
+        for (int k = 0; k < 9; ++k) cur[k] = cand.v[k];
       }
     }
   }
@@ -1462,6 +1489,8 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     out->workgroups_per_cu = (int32_t)ctx->tune_wgs;
     out->plane_order = (int32_t)ctx->tune_order;
     out->ensemble_plane_objects = (int32_t)ctx->tune_ens_plane;
+    out->row_cells = ctx->tiles.half_rows ? 2 : 1;
+    out->bucket_cells = ctx->tiles.half_buckets ? 2 : 1;
     out->scan_us = best_us;
     out->default_us = default_us;
     out->tried = tried;
@@ -1499,7 +1528,8 @@ int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *t) {
                   (t->tile_size == 4096 || t->tile_size == 8192 || t->tile_size == 16384) && t->chunks >= 1 &&
                   t->chunks <= 32 && t->workgroups_per_cu >= 0 && t->workgroups_per_cu <= 8 &&
                   (t->plane_order == 0 || t->plane_order == 1) && t->ensemble_plane_objects >= 0 &&
-                  t->ensemble_plane_objects <= 65536;
+                  t->ensemble_plane_objects <= 65536 && t->row_cells >= 0 && t->row_cells <= 2 &&
+                  t->bucket_cells >= 0 && t->bucket_cells <= 2;
   if (!ok) return ctx->fail(LADDS_E_INVALID, "tuning values outside the tuner's range");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   ctx->tune_cell_km = t->cell_size_km;
@@ -1511,6 +1541,8 @@ int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *t) {
   ctx->tune_wgs = t->workgroups_per_cu;
   ctx->tune_order = t->plane_order;
   ctx->tune_ens_plane = t->ensemble_plane_objects;
+  ctx->tune_rows = t->row_cells;
+  ctx->tune_bcells = t->bucket_cells;
   if (ctx->n == 0) return LADDS_OK;  // applied at the next ladds_set_objects
   std::vector<double> rx(ctx->n);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

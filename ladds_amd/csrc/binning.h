@@ -64,13 +64,20 @@ __device__ __forceinline__ CellInfo cell_of(double x, double y, double z, const 
   return c;
 }
 
-// Folded in-plane bucket (the low by + bz bits of cell_of(...).bucket) from y and z only.
-__device__ __forceinline__ uint32_t yz_bucket(double y, double z, const GridParams &g) {
-  uint32_t ly, lz;
-  const int32_t iy = cell_axis(y, g.inv_cell, ly);
-  const int32_t iz = cell_axis(z, g.inv_cell, lz);
+// Folded in-plane bucket of the plane tiles from y and z only: rows of height 1 / inv_cell_y
+// along y and buckets of width 1 / inv_cell_z along z (each the cell size, or twice it).
+// low_y (low_z) = 1 when the point lies in the lower half of its row (bucket) along y (z).
+__device__ __forceinline__ uint32_t yz_bucket(double y, double z, const GridParams &g, uint32_t &low_y,
+                                              uint32_t &low_z) {
+  const int32_t iy = cell_axis(y, g.inv_cell_y, low_y);
+  const int32_t iz = cell_axis(z, g.inv_cell_z, low_z);
   const uint32_t my = (1u << g.by) - 1u, mz = (1u << g.bz) - 1u;
   return (((uint32_t)iy & my) << g.bz) | ((uint32_t)iz & mz);
+}
+
+__device__ __forceinline__ uint32_t yz_bucket(double y, double z, const GridParams &g) {
+  uint32_t ly, lz;
+  return yz_bucket(y, z, g, ly, lz);
 }
 
 }  // namespace ladds

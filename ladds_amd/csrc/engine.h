@@ -39,6 +39,8 @@ static_assert(sizeof(SubstepEnv<float>) == 20 * 4, "env layout");
 struct GridParams {
   double inv_cell;       // 1 / cell size in y and z [1/km]
   double inv_cell_x;     // 1 / cell size in x (the plane width) [1/km], >= cell size
+  double inv_cell_y;     // plane tiles: 1 / bucket row height in y [1/km], the cell size or twice it
+  double inv_cell_z;     // plane tiles: 1 / bucket width in z [1/km], the cell size or twice it
   uint32_t bx, by, bz;   // log2 of Kx, Ky, Kz
   uint32_t nbuckets;     // Kx * Ky * Kz
 };
@@ -72,6 +74,13 @@ struct TileParams {
   // The plane pairs (scenario * nplanes + plane) in dispatch order (nscen * nplanes
   // entries), or null for index order.
   const uint32_t *plane_order;
+  // 1: bucket rows are two cells tall (GridParams::inv_cell_y), and an object scans two rows
+  // of the next plane, chosen by the half of its row it lies in; 0: rows one cell tall,
+  // three rows of the next plane.
+  uint32_t half_rows;
+  // 1: buckets are two cells wide along z (GridParams::inv_cell_z), and an object scans two
+  // buckets of a row, chosen by the half of its bucket it lies in; 0: three buckets.
+  uint32_t half_buckets;
 };
 
 // Sorted record of the plain FP64 form and of the split form: the position the

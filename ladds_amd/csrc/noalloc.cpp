@@ -147,6 +147,8 @@ extern "C" int ladds_conjunction_digest_noalloc(const ladds_device_objects *objs
   GridParams g{};
   g.inv_cell = 1.0 / (radius * (1.0 + 1e-9));
   g.inv_cell_x = g.inv_cell;
+  g.inv_cell_y = g.inv_cell;  // buckets of one cell (tp.half_rows = tp.half_buckets = 0)
+  g.inv_cell_z = g.inv_cell;
   g.bx = kPlaneBitsX;
   const uint32_t pb = std::min<uint32_t>(12u, std::max<uint32_t>(4u, ceil_log2(N / (1u << kPlaneBitsX) + 1)));
   g.by = pb / 2;

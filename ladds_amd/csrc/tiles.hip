@@ -13,7 +13,9 @@
 //   pairs      one workgroup per plane: gathers the plane's records and those of the
 //              next plane from every tile, counting-sorts both by bucket in LDS, and
 //              scans the 3x3x3 cells around each record from LDS (cells of side >= R,
-//              so a plane is one cell thick and holds half the objects of a 2R plane).
+//              so a plane is one cell thick and holds half the objects of a 2R plane;
+//              buckets two cells tall or wide are scanned two at a time, on the side of
+//              the half of the bucket the record lies in).
 //              Pairs inside the plane are taken from the lower sorted position, pairs
 //              with the next plane from the plane below, so every pair within the
 //              search radius is seen exactly once.  Candidates go through the exact
@@ -214,7 +216,11 @@ __device__ __forceinline__ bool within(const RecOf<FORM> &a, const RecOf<FORM> &
 // around an object.  Within its own plane a pair is taken from the lower sorted
 // position; pairs with the next plane are all taken here (the next plane's own scan
 // never looks back), pairs with the previous plane by the previous plane's workgroup.
-// Per object: six rows (three y-rows of its own plane, three of the next); in each row
+// Per object: six rows (three y-rows of its own plane, three of the next), or, with rows
+// two cells tall (tp.half_rows), four (two of each plane, on the side of the half of its
+// row the object lies in: fewer rows of the same buckets, so fewer folded aliases to
+// reject); in each row three buckets, or two with buckets two cells wide along z
+// (tp.half_buckets), the ones on the side of the half of its bucket.  In each row
 // the buckets fz-1, fz, fz+1 (mod Kz) are one contiguous run of the sorted array plus,
 // where the fold wraps (fz = 0 or Kz-1), a one-bucket run at the other end of the row.
 // Both runs are read unconditionally (the second one empty when there is no wrap), so
@@ -241,23 +247,54 @@ __device__ __forceinline__ void scan_plane(const PairTileArgs &a, const RecOf<FO
     if (CHUNKED && me.idx % nchunk != chunk) continue;
     double x, y, z;
     binning_point_of(me, x, y, z);
-    const uint32_t lb = yz_bucket(y, z, g);
+    uint32_t ly, lz;  // 1: the record lies in the lower half of its row (bucket)
+    const uint32_t lb = yz_bucket(y, z, g, ly, lz);
     const uint32_t fy = (lb >> sy) & my, fz = lb & mz;
-    const uint32_t zlo = fz ? fz - 1u : 0u, zhi = fz < mz ? fz + 1u : mz;
-    const bool wraps = (fz == 0u) || (fz == mz);
+    // buckets [zlo, zhi] of each row: fz-1, fz, fz+1 (one cell wide) or, two cells wide,
+    // fz-1, fz (lower half) or fz, fz+1 (upper half); where that crosses the fold, the
+    // bucket at the other end of the row (zw) is scanned as a second run
+    uint32_t zlo, zhi;
+    bool wraps;
+    if (a.tp.half_buckets == 0u) {
+      zlo = fz ? fz - 1u : 0u;
+      zhi = fz < mz ? fz + 1u : mz;
+      wraps = (fz == 0u) || (fz == mz);
+    } else {
+      const bool wl = fz == 0u && lz != 0u, wh = fz == mz && lz == 0u;
+      zlo = wl ? 0u : fz - lz;
+      zhi = wh ? mz : fz + 1u - lz;
+      wraps = wl || wh;
+    }
     const uint32_t zw = fz ? 0u : mz;  // the wrapped bucket when `wraps`
+    const bool half = a.tp.half_rows != 0u;
 #pragma unroll 1
-    for (uint32_t kind = 0; kind < 6u; ++kind) {
+    for (uint32_t kind = 0; kind < (half ? 4u : 6u); ++kind) {
       const bool next_plane = (kind & 1u) != 0u;
       // Own-plane pairs: with one workgroup per plane each pair is taken from its lower
       // sorted position.  The in-LDS placement order within a bucket depends on atomic
       // timing, so chunks of one plane (each sorting the plane itself) cannot share that
       // rule: they take a pair from its lower object index instead.
       const bool by_index = CHUNKED && !next_plane;
-      // own row y-1 sorts entirely below this record unless the fold wraps it (fy = 0)
-      if (!CHUNKED && kind == 0u && fy != 0u) continue;
+      // the row scanned, as an offset d (mod Ky) from the record's own row
+      uint32_t d;
+      if (!half) {
+        // rows one cell tall: rows y-1, y, y+1 of both planes; own row y-1 sorts entirely
+        // below this record unless the fold wraps it (fy = 0)
+        if (!CHUNKED && kind == 0u && fy != 0u) continue;
+        d = (kind >> 1) - 1u;
+      } else if (next_plane || CHUNKED) {
+        // rows two cells tall: partners within R (<= half a row) lie in rows y-1, y for a
+        // record in the lower half of its row, y, y+1 for one in the upper half
+        d = (kind >> 1) - ly;
+      } else {
+        // own plane, lower sorted position: row y (kind 2) and the row on the record's
+        // side (kind 0), y+1 for the upper half, y-1 for the lower half, which sorts
+        // entirely below this record unless the fold wraps it (fy = 0)
+        if (kind == 0u && ly != 0u && fy != 0u) continue;
+        d = kind == 2u ? 0u : (ly ? ~0u : 1u);
+      }
       const uint32_t *cur = next_plane ? cur_halo : cur_own;
-      const uint32_t row = ((fy + (kind >> 1) - 1u) & my) << sy;
+      const uint32_t row = ((fy + d) & my) << sy;
       const uint32_t b_lo = row | zlo, b_w = row | zw;
       const uint32_t lo0 = cur[(int32_t)b_lo - 1], hi0 = cur[row | zhi];
       const uint32_t w0 = cur[(int32_t)b_w - 1], w1 = cur[b_w];

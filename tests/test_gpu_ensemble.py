@@ -162,7 +162,8 @@ def test_graph_replay_matches_eager_launches(ensemble):
 def test_ensemble_plane_sizes_keep_pairs(plane_objects):
     """Ensemble planes sized for more (or fewer) objects fold more (or fewer) distant x-slabs
     into one plane of a scenario, in plane order and in the heaviest-first dispatch order
-    over all scenarios' planes; each scenario's pair set still equals the oracle's."""
+    over all scenarios' planes, with buckets of one or two cells; each scenario's pair set
+    still equals the oracle's."""
     pop, off = _ensemble()
     cfg = dict(dt=10.0, components=la.KEP | la.J2, precision=la.SPLIT, cutoff_km=10.0,
                threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0)
@@ -170,7 +171,8 @@ def test_ensemble_plane_sizes_keep_pairs(plane_objects):
         eng.set_objects(pop)
         eng.set_scenarios(off)
         t = eng.autotune(repetitions=1)
-        eng.set_tuning(dict(t, ensemble_plane_objects=plane_objects, plane_order=int(plane_objects >= 256)))
+        eng.set_tuning(dict(t, ensemble_plane_objects=plane_objects, plane_order=int(plane_objects >= 256),
+                            row_cells=2 if plane_objects != 256 else 1, bucket_cells=2 if plane_objects == 32 else 1))
         eng.propagate(2)
         hits, ev = eng.conjunctions()
         r, v = eng.state()

@@ -459,6 +459,41 @@ def test_lds_and_arena_planes_in_one_scan(precision):
     assert cnt == oh.shape[0] and dig == ol.digest(oh)
 
 
+@pytest.mark.parametrize("rows,width", [(2, 1), (2, 2), (1, 2)])
+@pytest.mark.parametrize("plane_bits,chunks", [(10, 1), (4, 1), (7, 4), (4, 3)])
+@pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
+def test_two_cell_buckets_keep_the_pair_set(rows, width, plane_bits, chunks, precision):
+    """Buckets two cells tall (row_cells = 2: each object scans two rows of the next plane,
+    picked by the half of its row it lies in) and/or two cells wide (bucket_cells = 2: two
+    buckets of a row, by the half of its bucket), one workgroup per plane and chunked planes
+    (index rule), down to 4 buckets per plane (two rows or two buckets a row: the fold puts
+    one on both sides of the other).  A LEO population with planted pairs plus a
+    2,500-object cluster: pair list and digest equal the oracle's."""
+    pop = la.generate_population(1, 0x1ADD5 + 18, 40000, planted_pairs=300)
+    rng = np.random.default_rng(18)
+    c = np.array([pop.rx[0], pop.ry[0], pop.rz[0]])
+    for i in range(1, 2500):
+        pop.rx[i], pop.ry[i], pop.rz[i] = c + rng.uniform(-4.0, 4.0, 3)
+        pop.vx[i], pop.vy[i], pop.vz[i] = pop.vx[0], pop.vy[0], pop.vz[0] + rng.uniform(-1e-3, 1e-3)
+    cell = 10.11 if precision == la.SPLIT else 10.0 * (1 + 1e-8)  # >= the search radius
+    shape = {"cell_size_km": cell, "plane_width_km": cell, "plane_bits": plane_bits, "pair_threads": 256,
+             "tile_size": 8192, "chunks": chunks, "workgroups_per_cu": 0, "row_cells": rows,
+             "bucket_cells": width}
+    with engine(dt=10.0, precision=precision, cutoff_km=10.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
+                threshold_km=1.0) as eng:
+        eng.set_objects(pop)
+        eng.set_tuning(shape)
+        hits, ev = eng.conjunctions()
+        cnt, dig, ecnt, edig = eng.conjunction_digest()
+    oh, oe = ol.pairs(_oracle_objects(pop.r(), pop.v(), pop),
+                      ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0))
+    assert oh.shape[0] > 1000
+    _pair_equal(ol.sort_pairs(hits), oh)
+    _pair_equal(ol.sort_pairs(ev), oe)
+    assert cnt == oh.shape[0] and dig == ol.digest(oh)
+    assert ecnt == oe.shape[0] and edig == ol.digest(oe)
+
+
 @pytest.mark.parametrize("threads", [128, 256, 384, 512, 1024])
 @pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
 def test_every_workgroup_size_keeps_the_pair_set(threads, precision):
