@@ -58,7 +58,8 @@ BYTES = {
 # HBM bytes per launch at 1,048,576 objects (read + write), measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by a 256 MiB copy kernel, on the plane-tile shape the
 # autotuner picks for this workload (FP64: 512-thread pair workgroups, 4,096-object tiles;
-# split: 256 threads, 8,192-object tiles); the runs and the commands are in
+# split: 256 threads, 8,192-object tiles; both: buckets of 2 x 2 cells); the runs and the
+# commands are in
 # profiles/hbm_traffic_1M_full_model.md.
 # FP64 and FP32 VALU flops per object-step of the propagation kernel (full force model),
 # from rocprofv3 SQ_INSTS_VALU_FLOPS_{FP64,FP32} (+ the _TRANS counters), per wavefront
@@ -73,10 +74,10 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32 (non-packed issue is half of this
 MEASURED_TRAFFIC_1M = {
     ("propagate", "fp64"): (71.47 + 54.65) * 1e6,
     ("propagate", "split"): (67.27 + 54.65) * 1e6,
-    ("tile_partition", "fp64"): (29.43 + 35.94) * 1e6,
+    ("tile_partition", "fp64"): (29.43 + 35.93) * 1e6,
     ("tile_partition", "split"): (29.45 + 17.83) * 1e6,
-    ("pair_tiles", "fp64"): (56.09 + 0.06) * 1e6,
-    ("pair_tiles", "split"): (31.76 + 0.06) * 1e6,
+    ("pair_tiles", "fp64"): (56.35 + 0.06) * 1e6,
+    ("pair_tiles", "split"): (32.00 + 0.06) * 1e6,
 }
 
 
