@@ -67,6 +67,21 @@ uint32_t ceil_log2(uint64_t v) {
   return b;
 }
 
+// One buffer of the overlapped simulation loop: a step's state (r, v, activity), its cell
+// keys, its plane-sorted records and tile offsets and the pair kernel's scratch arena.
+struct PipeBuffer {
+  double *r[3] = {nullptr, nullptr, nullptr}, *v[3] = {nullptr, nullptr, nullptr};
+  int32_t *act = nullptr;
+  uint32_t *keys = nullptr;
+  size_t cap = 0;  // objects the state, keys and records hold
+  void *rec = nullptr;
+  uint32_t *tps = nullptr;
+  size_t tps_alloc = 0;
+  uint32_t *arena_top = nullptr;
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
 }  // namespace
 
 struct ladds_ctx {
@@ -150,22 +165,16 @@ struct ladds_ctx {
 
   StepState *d_step = nullptr;  // graph replay position (ladds_config.step_graph)
 
-  // Overlapped simulation: step k+1's propagation and partition run on `stream` while
-  // step k's pair scan runs on `stream2`.  The state (r, v, activity), the records, the
-  // tile offsets and the scratch arena are double-buffered; buffer 0 is `d` itself.
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_part[2] = {nullptr, nullptr}, ev_pair[2] = {nullptr, nullptr};
+  // Overlapped simulation.  The eager loop cycles through three buffers: step k+2's
+  // propagation (`stream`) runs beside step k+1's partition (`stream3`) and step k's pair
+  // scan (`stream2`).  The step graphs (ladds_config.step_graph) alternate buffers 0 and 1,
+  // the partition on `stream`.  Buffer 0 is the context's own arrays (`d`, `slot`, `rec`,
+  // `tps`, `scratch`, `arena_top`); pipe[1] and pipe[2] hold the others.
+  hipStream_t stream2 = nullptr, stream3 = nullptr;
+  hipEvent_t ev_prop[3] = {nullptr, nullptr, nullptr}, ev_part[3] = {nullptr, nullptr, nullptr},
+             ev_pair[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev_join = nullptr;  // end of ladds_simulate: stream waits for stream2
-  double *sb_r[3] = {nullptr, nullptr, nullptr}, *sb_v[3] = {nullptr, nullptr, nullptr};
-  int32_t *sb_act = nullptr;
-  size_t sb_cap = 0;
-  void *rec2 = nullptr;
-  size_t rec2_cap = 0;
-  uint32_t *tps2 = nullptr;
-  size_t tps2_alloc = 0;
-  uint32_t *arena_top2 = nullptr;
-  void *scratch2 = nullptr;
-  size_t scratch2_bytes = 0;
+  PipeBuffer pipe[3];
 
   // Staging for the output records (ladds_particle_records, ladds_log_records).
   void *xbuf = nullptr;
@@ -240,6 +249,7 @@ hipError_t timed(ladds_ctx *ctx, const char *name, F &&launch) {
 void resolve_profiles(ladds_ctx *ctx) {
   hipStreamSynchronize(ctx->stream);
   if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
+  if (ctx->stream3) hipStreamSynchronize(ctx->stream3);
   for (auto &kv : ctx->prof) {
     for (EventPair &ev : kv.second.used) {
       float ms = 0.f;
@@ -661,44 +671,47 @@ int launch_detection(ladds_ctx *ctx, bool binned, double dt_interp, bool digest,
   return LADDS_OK;
 }
 
-// Second state buffer and second records / offsets / arena for the overlapped loop.
-int ensure_overlap_buffers(ladds_ctx *ctx) {
-  if (ctx->sb_cap < ctx->cap) {
-    for (int k = 0; k < 3; ++k) {
-      hipFree(ctx->sb_r[k]);
-      hipFree(ctx->sb_v[k]);
-      ctx->sb_r[k] = ctx->sb_v[k] = nullptr;
+// Pipeline buffers 1 .. nbuf-1 of the overlapped loop, sized like buffer 0.
+int ensure_overlap_buffers(ladds_ctx *ctx, int nbuf) {
+  for (int b = 1; b < nbuf; ++b) {
+    PipeBuffer &pb = ctx->pipe[b];
+    if (pb.cap < ctx->cap) {
+      for (int k = 0; k < 3; ++k) {
+        hipFree(pb.r[k]);
+        hipFree(pb.v[k]);
+        pb.r[k] = pb.v[k] = nullptr;
+      }
+      hipFree(pb.act);
+      hipFree(pb.keys);
+      hipFree(pb.rec);
+      pb.act = nullptr;
+      pb.keys = nullptr;
+      pb.rec = nullptr;
+      pb.cap = 0;
+      for (int k = 0; k < 3; ++k) {
+        HIPCHK(ctx, dalloc(&pb.r[k], ctx->cap));
+        HIPCHK(ctx, dalloc(&pb.v[k], ctx->cap));
+      }
+      HIPCHK(ctx, dalloc(&pb.act, ctx->cap));
+      HIPCHK(ctx, dalloc(&pb.keys, ctx->cap));
+      HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&pb.rec), ctx->cap));
+      pb.cap = ctx->cap;
     }
-    hipFree(ctx->sb_act);
-    ctx->sb_act = nullptr;
-    ctx->sb_cap = 0;
-    for (int k = 0; k < 3; ++k) {
-      HIPCHK(ctx, dalloc(&ctx->sb_r[k], ctx->cap));
-      HIPCHK(ctx, dalloc(&ctx->sb_v[k], ctx->cap));
+    if (!pb.arena_top) HIPCHK(ctx, dalloc(&pb.arena_top, 1));
+    if (pb.tps_alloc < ctx->tps_alloc) {
+      hipFree(pb.tps);
+      pb.tps = nullptr;
+      pb.tps_alloc = 0;
+      HIPCHK(ctx, dalloc(&pb.tps, ctx->tps_alloc));
+      pb.tps_alloc = ctx->tps_alloc;
     }
-    HIPCHK(ctx, dalloc(&ctx->sb_act, ctx->cap));
-    ctx->sb_cap = ctx->cap;
-  }
-  if (ctx->rec2_cap < ctx->cap) {
-    hipFree(ctx->rec2);
-    ctx->rec2 = nullptr;
-    ctx->rec2_cap = 0;
-    HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&ctx->rec2), ctx->cap));
-    ctx->rec2_cap = ctx->cap;
-  }
-  if (ctx->tps2_alloc < ctx->tps_alloc) {
-    hipFree(ctx->tps2);
-    ctx->tps2 = nullptr;
-    ctx->tps2_alloc = 0;
-    HIPCHK(ctx, dalloc(&ctx->tps2, ctx->tps_alloc));
-    ctx->tps2_alloc = ctx->tps_alloc;
-  }
-  if (ctx->scratch2_bytes < ctx->scratch_bytes) {
-    hipFree(ctx->scratch2);
-    ctx->scratch2 = nullptr;
-    ctx->scratch2_bytes = 0;
-    HIPCHK(ctx, hipMalloc(&ctx->scratch2, ctx->scratch_bytes));
-    ctx->scratch2_bytes = ctx->scratch_bytes;
+    if (pb.scratch_bytes < ctx->scratch_bytes) {
+      hipFree(pb.scratch);
+      pb.scratch = nullptr;
+      pb.scratch_bytes = 0;
+      HIPCHK(ctx, hipMalloc(&pb.scratch, ctx->scratch_bytes));
+      pb.scratch_bytes = ctx->scratch_bytes;
+    }
   }
   return LADDS_OK;
 }
@@ -707,15 +720,51 @@ int ensure_overlap_buffers(ladds_ctx *ctx) {
 DeviceObjects state_view(const ladds_ctx *ctx, int b) {
   DeviceObjects v = ctx->d;
   if (b) {
-    v.rx = ctx->sb_r[0];
-    v.ry = ctx->sb_r[1];
-    v.rz = ctx->sb_r[2];
-    v.vx = ctx->sb_v[0];
-    v.vy = ctx->sb_v[1];
-    v.vz = ctx->sb_v[2];
-    v.activity = ctx->sb_act;
+    const PipeBuffer &pb = ctx->pipe[b];
+    v.rx = pb.r[0];
+    v.ry = pb.r[1];
+    v.rz = pb.r[2];
+    v.vx = pb.v[0];
+    v.vy = pb.v[1];
+    v.vz = pb.v[2];
+    v.activity = pb.act;
   }
   return v;
+}
+
+uint32_t *keys_of(ladds_ctx *ctx, int b) { return b ? ctx->pipe[b].keys : ctx->slot; }
+void *rec_of(ladds_ctx *ctx, int b) { return b ? ctx->pipe[b].rec : ctx->rec; }
+uint32_t *tps_of(ladds_ctx *ctx, int b) { return b ? ctx->pipe[b].tps : ctx->tps; }
+void *scratch_of(ladds_ctx *ctx, int b) { return b ? ctx->pipe[b].scratch : ctx->scratch; }
+uint32_t *arena_top_of(ladds_ctx *ctx, int b) { return b ? ctx->pipe[b].arena_top : ctx->arena_top; }
+
+// Propagate from state buffer `cur` into `nxt` (on `stream`), writing the cell keys of the
+// result when `keys`.  With `step_state` the environments start at its step + step_offset.
+int launch_prop_into(ladds_ctx *ctx, int cur, int nxt, size_t env_offset, int nsteps, bool keys, double burnup_r2,
+                     const StepState *step_state, uint32_t step_offset) {
+  const DeviceObjects in = state_view(ctx, cur), out = state_view(ctx, nxt);
+  double cdt[4], ddt[3];
+  for (int k = 0; k < 4; ++k) cdt[k] = ctx->c[k] * ctx->cfg.dt;
+  for (int k = 0; k < 3; ++k) ddt[k] = ctx->dcoef[k] * ctx->cfg.dt;
+  const hipError_t e = timed(ctx, "propagate", [&] {
+    return launch_propagate(ctx->form, in, ctx->env64 + 3 * env_offset, ctx->env32 + 3 * env_offset, nsteps,
+                            ctx->cfg.components, cdt, ddt, &ctx->grid, keys ? keys_of(ctx, nxt) : nullptr, nullptr,
+                            ctx->slot, burnup_r2, step_state, ctx->stream, &out, step_offset,
+                            &ctx->counters->burnups);
+  });
+  if (e != hipSuccess) return ctx->hipfail(e, "propagation launch");
+  return LADDS_OK;
+}
+
+// Partition state buffer b into its records (on `stream`), staging at most stage_bytes of
+// records in LDS per pass (0: the whole tile when it fits).
+int launch_part_of(ladds_ctx *ctx, int b, hipStream_t stream, size_t stage_bytes = 0) {
+  const hipError_t e = timed_on(ctx, "tile_partition", stream, [&] {
+    return launch_tile_partition(ctx->form, state_view(ctx, b), keys_of(ctx, b), ctx->tiles, rec_of(ctx, b),
+                                 tps_of(ctx, b), arena_top_of(ctx, b), stream, stage_bytes);
+  });
+  if (e != hipSuccess) return ctx->hipfail(e, "partition launch");
+  return LADDS_OK;
 }
 
 // Propagate from state buffer `cur` into 1 - cur and, when `partition`, partition the result
@@ -723,23 +772,9 @@ DeviceObjects state_view(const ladds_ctx *ctx, int b) {
 int launch_prop_part(ladds_ctx *ctx, int cur, size_t env_offset, int nsteps, bool partition, double burnup_r2,
                      const StepState *step_state, uint32_t step_offset) {
   const int nxt = 1 - cur;
-  const DeviceObjects in = state_view(ctx, cur), out = state_view(ctx, nxt);
-  double cdt[4], ddt[3];
-  for (int k = 0; k < 4; ++k) cdt[k] = ctx->c[k] * ctx->cfg.dt;
-  for (int k = 0; k < 3; ++k) ddt[k] = ctx->dcoef[k] * ctx->cfg.dt;
-  hipError_t e = timed(ctx, "propagate", [&] {
-    return launch_propagate(ctx->form, in, ctx->env64 + 3 * env_offset, ctx->env32 + 3 * env_offset, nsteps,
-                            ctx->cfg.components, cdt, ddt, &ctx->grid, partition ? ctx->slot : nullptr, nullptr,
-                            ctx->slot, burnup_r2, step_state, ctx->stream, &out, step_offset,
-                            &ctx->counters->burnups);
-  });
-  if (e == hipSuccess && partition)
-    e = timed(ctx, "tile_partition", [&] {
-      return launch_tile_partition(ctx->form, out, ctx->slot, ctx->tiles, nxt ? ctx->rec2 : ctx->rec,
-                                   nxt ? ctx->tps2 : ctx->tps, nxt ? ctx->arena_top2 : ctx->arena_top, ctx->stream);
-    });
-  if (e != hipSuccess) return ctx->hipfail(e, "propagation launch");
-  return LADDS_OK;
+  int rc = launch_prop_into(ctx, cur, nxt, env_offset, nsteps, partition, burnup_r2, step_state, step_offset);
+  if (rc == LADDS_OK && partition) rc = launch_part_of(ctx, nxt, ctx->stream);
+  return rc;
 }
 
 // Pair scan of state buffer b (records partitioned into buffer b) on `stream`, with the
@@ -750,26 +785,41 @@ int launch_pairs_of(ladds_ctx *ctx, int b, double dt_interp, bool digest, uint64
   hipError_t e = spawn_pre(ctx, spawn_span, stream);
   if (e == hipSuccess)
     e = timed_on(ctx, "pair_tiles", stream, [&] {
-      return launch_pair_tiles(ctx->form, digest, state_view(ctx, b), ctx->grid, ctx->tiles, fp,
-                               b ? ctx->rec2 : ctx->rec, b ? ctx->tps2 : ctx->tps, b ? ctx->scratch2 : ctx->scratch,
-                               b ? ctx->arena_top2 : ctx->arena_top, ctx->log, ctx->log_cap, ctx->log_count,
-                               ctx->counters, iteration, step_state, stream);
+      return launch_pair_tiles(ctx->form, digest, state_view(ctx, b), ctx->grid, ctx->tiles, fp, rec_of(ctx, b),
+                               tps_of(ctx, b), scratch_of(ctx, b), arena_top_of(ctx, b), ctx->log, ctx->log_cap,
+                               ctx->log_count, ctx->counters, iteration, step_state, stream);
     });
   if (e == hipSuccess) e = spawn_post(ctx, spawn_span, true, stream);
   if (e != hipSuccess) return ctx->hipfail(e, "pair scan launch");
   return LADDS_OK;
 }
 
-// One overlapped iteration: propagate from buffer `cur` into buffer 1 - cur (stream),
-// and when `detect`, partition it (stream) and scan its pairs (stream2).
+// LDS window of the overlapped loop's partitions of FP32 records.  A workgroup staging a
+// whole 8,192-record tile takes 136 KB of LDS and waits for a CU the pair kernel has left;
+// staging 32 KB per pass (four passes) it runs beside three split pair workgroups.  Measured
+// at 1M objects: 1.41e10 -> 1.47e10 object-steps/s (16, 48 and 64 KB: 1.45-1.46e10).  FP64
+// records keep the whole tile: two 80 KB FP64 pair workgroups leave no such room on a CU, and
+// the extra passes only cost (1.02e10 -> 0.98e10 with 32 or 64 KB).
+constexpr size_t kPipelineStageBytes = 32 * 1024;
+
+// One iteration of the eager overlapped loop: propagate buffer `cur` into nxt = cur + 1
+// (mod 3) on `stream`; when `detect`, partition nxt on `stream3` once the propagation is
+// done, and scan its pairs on `stream2` once the partition is done.  Buffer nxt was last
+// read by the partition and the pair scan of the step three iterations back (`pending_next`):
+// the propagation waits for that pair scan, which itself followed its partition.  So step
+// k+2's propagation, step k+1's partition and step k's pair scan can all run at once, and
+// the partition (bound by HBM traffic) no longer holds up the next propagation.
 int overlapped_iteration(ladds_ctx *ctx, int cur, bool pending_next, size_t env_offset, int nsteps, bool detect,
                          double burnup_r2, double dt_interp, bool digest, uint64_t iteration, uint64_t spawn_span) {
-  const int nxt = 1 - cur;
-  // buffer nxt was last read by a pair scan on stream2
+  const int nxt = (cur + 1) % 3;
   if (pending_next) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_pair[nxt], 0));
-  int rc = launch_prop_part(ctx, cur, env_offset, nsteps, detect, burnup_r2, nullptr, 0);
+  int rc = launch_prop_into(ctx, cur, nxt, env_offset, nsteps, detect, burnup_r2, nullptr, 0);
   if (rc != LADDS_OK || !detect) return rc;
-  HIPCHK(ctx, hipEventRecord(ctx->ev_part[nxt], ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev_prop[nxt], ctx->stream));
+  HIPCHK(ctx, hipStreamWaitEvent(ctx->stream3, ctx->ev_prop[nxt], 0));
+  rc = launch_part_of(ctx, nxt, ctx->stream3, ctx->form == kFormFp64 ? 0 : kPipelineStageBytes);
+  if (rc != LADDS_OK) return rc;
+  HIPCHK(ctx, hipEventRecord(ctx->ev_part[nxt], ctx->stream3));
   HIPCHK(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_part[nxt], 0));
   rc = launch_pairs_of(ctx, nxt, dt_interp, digest, iteration, nullptr, ctx->stream2, spawn_span);
   if (rc != LADDS_OK) return rc;
@@ -904,13 +954,14 @@ int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
   hipError_t e = hipSetDevice(ctx->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
-  for (int b = 0; b < 2 && e == hipSuccess; ++b) {
-    e = hipEventCreateWithFlags(&ctx->ev_part[b], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking);
+  for (int b = 0; b < 3 && e == hipSuccess; ++b) {
+    e = hipEventCreateWithFlags(&ctx->ev_prop[b], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_part[b], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_pair[b], hipEventDisableTiming);
   }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_rb), sizeof(HostReadback), 0);
-  if (e == hipSuccess) e = dalloc(&ctx->arena_top2, 1);
   if (e == hipSuccess) e = upload_atmosphere();
   if (e == hipSuccess) e = configure_tile_kernels();
   if (e == hipSuccess) e = dalloc(&ctx->arena_top, 1);
@@ -937,16 +988,21 @@ int ladds_destroy(ladds_ctx *ctx) {
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
-  for (int k = 0; k < 3; ++k) {
-    hipFree(ctx->sb_r[k]);
-    hipFree(ctx->sb_v[k]);
+  if (ctx->stream3) hipStreamSynchronize(ctx->stream3);
+  for (PipeBuffer &pb : ctx->pipe) {
+    for (int k = 0; k < 3; ++k) {
+      hipFree(pb.r[k]);
+      hipFree(pb.v[k]);
+    }
+    hipFree(pb.act);
+    hipFree(pb.keys);
+    hipFree(pb.rec);
+    hipFree(pb.tps);
+    hipFree(pb.arena_top);
+    hipFree(pb.scratch);
   }
-  hipFree(ctx->sb_act);
-  hipFree(ctx->rec2);
-  hipFree(ctx->tps2);
-  hipFree(ctx->arena_top2);
-  hipFree(ctx->scratch2);
-  for (int b = 0; b < 2; ++b) {
+  for (int b = 0; b < 3; ++b) {
+    if (ctx->ev_prop[b]) hipEventDestroy(ctx->ev_prop[b]);
     if (ctx->ev_part[b]) hipEventDestroy(ctx->ev_part[b]);
     if (ctx->ev_pair[b]) hipEventDestroy(ctx->ev_pair[b]);
   }
@@ -956,6 +1012,7 @@ int ladds_destroy(ladds_ctx *ctx) {
   if (ctx->h_x) hipHostFree(ctx->h_x);
   hipFree(ctx->d_spawn);
   if (ctx->stream2) hipStreamDestroy(ctx->stream2);
+  if (ctx->stream3) hipStreamDestroy(ctx->stream3);
   free_objects(ctx->d);
   free_objects(ctx->d_alt);
   hipFree(ctx->count);
@@ -1695,9 +1752,10 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     const int rc = alloc_env(ctx);
     if (rc != LADDS_OK) return rc;
   }
-  // Step graphs (ladds_config.step_graph, detection every step): see capture_step_graph.
-  // Otherwise the eager loop overlaps propagation + partition (stream) with the pair scan
-  // (stream2).  Both ping-pong between the two state buffers.
+  // Step graphs (ladds_config.step_graph, detection every step): see capture_step_graph;
+  // they alternate buffers 0 and 1.  Otherwise the eager loop cycles through three buffers,
+  // overlapping propagation (stream), partition (stream3) and pair scan (stream2) of
+  // consecutive steps: see overlapped_iteration.
   const bool use_graph = ctx->cfg.step_graph != 0 && tpcd == 1 && ctx->variant == kVariantPlaneTiles &&
                          !ctx->profiling && ctx->n > 0 && nsteps > 0;
   const bool use_overlap = !use_graph && ctx->variant == kVariantPlaneTiles && ctx->n > 0 && nsteps > 0;
@@ -1711,10 +1769,10 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_spawn, &ctx->h_rb->spawn, sizeof(SpawnState), hipMemcpyHostToDevice,
                                ctx->stream));
   }
-  int cur = 0;                       // state buffer holding the latest state
-  bool pending[2] = {false, false};  // a pair scan reading buffer b may still run on stream2
+  int cur = 0;                              // state buffer holding the latest state
+  bool pending[3] = {false, false, false};  // a pair scan reading buffer b may still run on stream2
   if (use_overlap || use_graph) {
-    const int rc = ensure_overlap_buffers(ctx);
+    const int rc = ensure_overlap_buffers(ctx, use_graph ? 2 : 3);
     if (rc != LADDS_OK) return rc;
   }
   hipGraphExec_t graphs[2] = {nullptr, nullptr};
@@ -1784,11 +1842,12 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
       if (detect) next_acc = spawn ? it + 1 : next_acc;
       int rc;
       if (use_overlap) {
-        rc = overlapped_iteration(ctx, cur, pending[1 - cur], (size_t)s, k, detect, burnup_r2,
+        const int nxt = (cur + 1) % 3;
+        rc = overlapped_iteration(ctx, cur, pending[nxt], (size_t)s, k, detect, burnup_r2,
                                   ctx->cfg.dt * (double)tpcd, digest, it, span);
         if (rc == LADDS_OK) {
-          if (detect) pending[1 - cur] = true;
-          cur = 1 - cur;
+          pending[nxt] = detect;
+          cur = nxt;
         }
       } else {
         rc = launch_propagate_steps(ctx, (size_t)s, k, detect, burnup_r2);
@@ -1819,14 +1878,15 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
                                ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   drop_graphs();
-  if ((use_overlap || use_graph) && cur == 1) {  // the final state is in the second buffer: make it the context's own
-    std::swap(ctx->d.rx, ctx->sb_r[0]);
-    std::swap(ctx->d.ry, ctx->sb_r[1]);
-    std::swap(ctx->d.rz, ctx->sb_r[2]);
-    std::swap(ctx->d.vx, ctx->sb_v[0]);
-    std::swap(ctx->d.vy, ctx->sb_v[1]);
-    std::swap(ctx->d.vz, ctx->sb_v[2]);
-    std::swap(ctx->d.activity, ctx->sb_act);
+  if ((use_overlap || use_graph) && cur != 0) {  // the final state is in another buffer: make it the context's own
+    PipeBuffer &pb = ctx->pipe[cur];
+    std::swap(ctx->d.rx, pb.r[0]);
+    std::swap(ctx->d.ry, pb.r[1]);
+    std::swap(ctx->d.rz, pb.r[2]);
+    std::swap(ctx->d.vx, pb.v[0]);
+    std::swap(ctx->d.vy, pb.v[1]);
+    std::swap(ctx->d.vz, pb.v[2]);
+    std::swap(ctx->d.activity, pb.act);
   }
   const ScanCounters sc = ctx->h_rb->sc;
   const unsigned long long cnt = ctx->h_rb->log_count;

@@ -33,8 +33,11 @@ hipError_t launch_pair_scan(int form, bool digest, const DeviceObjects &o, const
 
 // Plane-tile pipeline (tiles.hip)
 hipError_t launch_bin_keys(int form, const DeviceObjects &o, const GridParams &g, uint32_t *keys, hipStream_t s);
+// stage_bytes: most LDS bytes of records a partition workgroup stages per pass (0: the whole
+// tile when it fits); a smaller window lets the partition share CUs with other kernels.
 hipError_t launch_tile_partition(int form, const DeviceObjects &o, const uint32_t *keys, const TileParams &tp,
-                                 void *items, uint32_t *tps, uint32_t *arena_top, hipStream_t s);
+                                 void *items, uint32_t *tps, uint32_t *arena_top, hipStream_t s,
+                                 size_t stage_bytes = 0);
 hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, const GridParams &g,
                              const TileParams &tp, const FunctorParams &fp, const void *items, const uint32_t *tps,
                              void *scratch, uint32_t *arena_top, LoggedPair *log, uint64_t log_cap,

@@ -24,6 +24,8 @@
 //   arena instead (same code, global storage).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "binning.h"
 #include "launch.h"
 #include "pairs.h"
@@ -559,12 +561,15 @@ hipError_t launch_bin_keys(int form, const DeviceObjects &o, const GridParams &g
 
 template <typename REC, int ITEMS>
 void launch_partition_items(const DeviceObjects &o, const uint32_t *keys, const TileParams &tp, REC *out,
-                            uint32_t *tps, uint32_t *arena_top, hipStream_t s) {
+                            uint32_t *tps, uint32_t *arena_top, hipStream_t s, size_t stage_bytes) {
   const dim3 grid(tp.ntiles), block(kPartThreads);
   const size_t head = sizeof(uint32_t) * partition_header_words(tp.nplanes);
-  // staging capacity: the whole tile when it fits the LDS, else the largest multiple of
-  // 1024 records that does (two or more passes)
+  // staging capacity: the whole tile when it fits the LDS (and the caller's window), else the
+  // largest multiple of 1024 records that does (two or more passes).  With at most 2^14
+  // planes the header leaves room for 3072 FP64 records, so a pass holds >= 1024 records.
   size_t cap = (size_t)ITEMS * kPartThreads;
+  if (stage_bytes && sizeof(REC) * cap > stage_bytes)
+    cap = std::max<size_t>(1024, stage_bytes / sizeof(REC) / 1024 * 1024);
   if (head + sizeof(REC) * cap > kPairTileLdsBytes) cap = (kPairTileLdsBytes - head) / sizeof(REC) / 1024 * 1024;
   hipLaunchKernelGGL((k_tile_partition<REC, ITEMS>), grid, block, head + sizeof(REC) * cap, s, o.rx, o.ry, o.rz, o.n,
                      keys, tp, out, tps, arena_top, (uint32_t)cap);
@@ -572,17 +577,17 @@ void launch_partition_items(const DeviceObjects &o, const uint32_t *keys, const 
 
 template <typename REC>
 hipError_t launch_tile_partition_as(const DeviceObjects &o, const uint32_t *keys, const TileParams &tp, void *items,
-                                    uint32_t *tps, uint32_t *arena_top, hipStream_t s) {
+                                    uint32_t *tps, uint32_t *arena_top, hipStream_t s, size_t stage_bytes) {
   REC *out = static_cast<REC *>(items);
   switch (tp.tile_size) {
     case 4096:
-      launch_partition_items<REC, 4>(o, keys, tp, out, tps, arena_top, s);
+      launch_partition_items<REC, 4>(o, keys, tp, out, tps, arena_top, s, stage_bytes);
       break;
     case 8192:
-      launch_partition_items<REC, 8>(o, keys, tp, out, tps, arena_top, s);
+      launch_partition_items<REC, 8>(o, keys, tp, out, tps, arena_top, s, stage_bytes);
       break;
     case 16384:
-      launch_partition_items<REC, 16>(o, keys, tp, out, tps, arena_top, s);
+      launch_partition_items<REC, 16>(o, keys, tp, out, tps, arena_top, s, stage_bytes);
       break;
     default:
       return hipErrorInvalidValue;
@@ -591,10 +596,11 @@ hipError_t launch_tile_partition_as(const DeviceObjects &o, const uint32_t *keys
 }
 
 hipError_t launch_tile_partition(int form, const DeviceObjects &o, const uint32_t *keys, const TileParams &tp,
-                                 void *items, uint32_t *tps, uint32_t *arena_top, hipStream_t s) {
+                                 void *items, uint32_t *tps, uint32_t *arena_top, hipStream_t s, size_t stage_bytes) {
   if (o.n == 0) return hipSuccess;
-  if (form == kFormFp64) return launch_tile_partition_as<RecordF64>(o, keys, tp, items, tps, arena_top, s);
-  return launch_tile_partition_as<RecordF32>(o, keys, tp, items, tps, arena_top, s);
+  if (form == kFormFp64)
+    return launch_tile_partition_as<RecordF64>(o, keys, tp, items, tps, arena_top, s, stage_bytes);
+  return launch_tile_partition_as<RecordF32>(o, keys, tp, items, tps, arena_top, s, stage_bytes);
 }
 
 template <int THREADS>

@@ -300,6 +300,56 @@ def test_simulate_matches_oracle_step_by_step(variant):
     assert total >= 48
 
 
+def _cluster_population(seed, n=40000, cluster=2500):
+    """A LEO population with planted pairs plus `cluster` objects within a 4 km cube moving
+    almost together: thousands of conjunctions every step."""
+    pop = la.generate_population(1, 0x1ADD5 + seed, n, planted_pairs=200)
+    rng = np.random.default_rng(seed)
+    c = np.array([pop.rx[0], pop.ry[0], pop.rz[0]])
+    for i in range(1, cluster):
+        pop.rx[i], pop.ry[i], pop.rz[i] = c + rng.uniform(-4.0, 4.0, 3)
+        pop.vx[i], pop.vy[i], pop.vz[i] = pop.vx[0], pop.vy[0], pop.vz[0] + rng.uniform(-1e-3, 1e-3)
+    return pop
+
+
+def _d2_bits(x):
+    return int(np.array(x, np.float64).view(np.int64))
+
+
+@pytest.mark.parametrize("steps,tpcd", [(30, 1), (31, 1), (32, 1), (29, 2), (40, 3)])
+@pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
+def test_simulate_pipeline_equals_step_by_step(steps, tpcd, precision):
+    """ladds_simulate's eager loop cycles through three buffers, with the propagation of step
+    k+2, the partition of step k+1 and the pair scan of step k in flight together.  Its pair
+    log (iteration, ids, d^2 bits, evaded) and its final state equal, bit for bit, those of
+    the same steps run one call at a time (ladds_propagate, then ladds_conjunctions at each
+    detection iteration).  The step counts leave the final state in each of the three
+    buffers; detection every 1, 2 and 3 iterations."""
+    pop = _cluster_population(19)
+    cfg = dict(dt=10.0, components=la.ALL_COMPONENTS, precision=precision, cutoff_km=10.0,
+               threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, log_capacity=1 << 22)
+    with engine(**cfg) as eng:
+        eng.set_objects(pop)
+        stats = eng.simulate(steps, first_iteration=0, tpcd=tpcd)
+        log = eng.fetch_log()
+        r1, v1 = eng.state()
+    ref = []
+    with engine(**cfg) as eng:
+        eng.set_objects(pop)
+        for it in range(steps):
+            eng.propagate(1)
+            if it % tpcd == 0:
+                hits, ev = eng.conjunctions(dt_interp=10.0 * tpcd)
+                for arr, evaded in ((hits, 0), (ev, 1)):
+                    ref += [(it, int(p["id_a"]), int(p["id_b"]), _d2_bits(p["d2"]), evaded) for p in arr]
+        r0, v0 = eng.state()
+    got = [(int(e["iteration"]), int(e["pair"]["id_a"]), int(e["pair"]["id_b"]), _d2_bits(e["pair"]["d2"]),
+            int(e["evaded"])) for e in log]
+    assert stats["logged"] == len(got) > 1000 * ((steps + tpcd - 1) // tpcd)
+    assert sorted(got) == sorted(ref)
+    assert np.array_equal(r1, r0) and np.array_equal(v1, v0)
+
+
 def test_timesteps_per_collision_detection(variant):
     """SimulationTest.cpp:144-193: head-on pair, tpcd = 2 -> 0 / 0 / 1 conjunctions."""
     re = 6378.1363
