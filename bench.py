@@ -106,6 +106,9 @@ def parse():
     p.add_argument("--tuning", metavar="JSON",
                    help="apply these plane-tile shapes instead of autotuning: a JSON object per form, as the "
                         "'tuning' field of an earlier run's line (profiler runs then measure the tuned shape)")
+    p.add_argument("--clustered-tuning", metavar="JSON",
+                   help="apply this plane-tile shape (one JSON object, both forms) in the clustered leg instead "
+                        "of autotuning it")
     p.add_argument("--profile-mode", choices=("overlapped", "serial"), default="overlapped",
                    help="profiled pass: the overlapped loop (default), or one kernel at a time (propagate one "
                         "step, then scan its pairs)")
@@ -355,11 +358,12 @@ def clustered_cpu_baseline(la, pop, ids, r, v, gpu_digest):
             "digest_equals_gpu": bool(want == gpu_digest)}
 
 
-def clustered_leg(la, steps, cpu_check=True):
+def clustered_leg(la, steps, cpu_check=True, pinned=None):
     """8,388,608 objects, 60 % replicated constellation slots (+-50 m) and 40 % background;
     split precision, dt = 10 s, cutoff 5 km, 0.5 km threshold, pair output as count + digest;
-    the autotuner picks cell size and tile shape.  The FP64 and split forms' digests of the
-    initial state are compared.  Timed over `steps` steps of the 1,000-step configuration."""
+    the autotuner picks cell size and tile shape (or `pinned` applies one).  The FP64 and
+    split forms' digests of the initial state are compared.  Timed over `steps` steps of the
+    1,000-step configuration."""
     n = 8388608
     pop = la.generate_population(2, 0x1ADD5 + 2, n)
     out = {"workload": f"{n} objects clustered (constellation shells), split precision, dt 10 s, cutoff 5 km, "
@@ -371,7 +375,11 @@ def clustered_leg(la, steps, cpu_check=True):
                               log_capacity=1 << 16)
         with la.Engine(cfg) as eng:
             eng.set_objects(pop)
-            tuning = eng.autotune(repetitions=1)
+            if pinned:
+                eng.set_tuning(pinned)
+                tuning = dict(pinned)
+            else:
+                tuning = eng.autotune(repetitions=1)
             digests[name] = eng.conjunction_digest(10.0)
             if name == "split":
                 eng.synchronize()
@@ -534,7 +542,9 @@ def main():
         print(f"# ensemble: {e['object_steps_per_s']:.4g} object-steps/s (eager {e['eager_object_steps_per_s']:.4g}),"
               f" graph==eager {e['graph_equals_eager']}, collisions {e['collisions']}", file=sys.stderr, flush=True)
     if not a.no_clustered:
-        out["clustered"] = clustered_leg(la, a.clustered_steps, not a.no_cpu_baseline)
+        ctext = (a.clustered_tuning or "").strip()
+        cpinned = (json.loads(ctext if ctext.startswith("{") else Path(ctext).read_text()) if ctext else None)
+        out["clustered"] = clustered_leg(la, a.clustered_steps, not a.no_cpu_baseline, cpinned)
         print(f"# clustered: {json.dumps(out['clustered'])}", file=sys.stderr, flush=True)
     if dropin is not None:
         out["dropin"] = dropin
