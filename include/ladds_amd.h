@@ -294,12 +294,15 @@ typedef struct ladds_tuning {
                          /* half of its row it lies in); 0 is read as 1                  */
   int32_t bucket_cells;  /* bucket width along z in cells: 1 (three buckets of a row) or  */
                          /* 2 (two, on the side of the half of its bucket); 0 is read as 1 */
+  int32_t persistent;    /* pair kernel: 0 one workgroup per plane pair; 1 as many         */
+                         /* workgroups as the CUs hold, taking plane pairs from a queue   */
+                         /* per XCD until none is left                                     */
 } ladds_tuning;
 
 LADDS_API int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuning *out);
 /* Apply a shape that ladds_autotune chose earlier (cell_size_km, plane_width_km,
  * plane_bits, pair_threads, tile_size, chunks, workgroups_per_cu, plane_order,
- * ensemble_plane_objects, row_cells, bucket_cells; the timing fields are
+ * ensemble_plane_objects, row_cells, bucket_cells, persistent; the timing fields are
  * ignored) without timing anything -- the counterpart of running LADDS with the
  * calibrated AutoPas configuration that tuning mode writes out (Simulation.cpp:404-407,
  * dumpCalibratedConfig).  A value outside the tuner's range is LADDS_E_INVALID. */

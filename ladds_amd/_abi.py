@@ -113,6 +113,7 @@ class Tuning(C.Structure):
         ("ensemble_plane_objects", C.c_int32),
         ("row_cells", C.c_int32),
         ("bucket_cells", C.c_int32),
+        ("persistent", C.c_int32),
     ]
 
 

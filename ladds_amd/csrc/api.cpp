@@ -90,6 +90,7 @@ struct ladds_ctx {
   std::string err;
   hipStream_t stream = nullptr;
   int device = 0;
+  int cus = 0;  // compute units of the device
 
   size_t n = 0, cap = 0;
   DeviceObjects d{};
@@ -162,6 +163,7 @@ struct ladds_ctx {
   int tune_ens_plane = 0;  // ensembles: objects per x-plane the planes are sized for (0: 64)
   int tune_rows = 0;       // bucket row height along y in cells: 2 or 1 (0: 1)
   int tune_bcells = 0;     // bucket width along z in cells: 2 or 1 (0: 1)
+  int tune_persistent = 0; // 1: pair kernel as persistent workgroups with per-XCD queues
 
   StepState *d_step = nullptr;  // graph replay position (ladds_config.step_graph)
 
@@ -506,6 +508,15 @@ int setup_grid(ladds_ctx *ctx, size_t n, const double *rx) {
     size_t wgs = std::min(wave_cap, kPairTileLdsBytes / (hdr + need * rec + kLdsMargin));
     if (ctx->tune_wgs >= 1 && ctx->tune_wgs <= 8) wgs = (size_t)ctx->tune_wgs;
     wgs = std::max<size_t>(1, wgs);
+    // persistent pair kernel: as many workgroups as the CUs hold at once (its own
+    // instantiation's registers and the LDS share; a multiple of the 8 XCDs, each with its
+    // share of the queues)
+    tp.persistent_groups = 0u;
+    if (tiles && ctx->tune_persistent == 1 && ctx->cus > 0) {
+      const int p_regs = pair_tiles_groups_per_cu(ctx->form, tp.pair_threads, true);
+      const size_t per_cu = p_regs > 0 ? std::min<size_t>(wgs, (size_t)p_regs) : wgs;
+      tp.persistent_groups = (uint32_t)(((size_t)ctx->cus * per_cu + 7) / 8 * 8);
+    }
     const size_t share = kPairTileLdsBytes / wgs;
     const size_t cap = share > hdr + kLdsMargin ? (share - hdr - kLdsMargin) / rec / 64 * 64 : 0;
     tp.cap_items = (uint32_t)std::min(fit, std::max<size_t>(cap, 256));
@@ -697,7 +708,7 @@ int ensure_overlap_buffers(ladds_ctx *ctx, int nbuf) {
       HIPCHK(ctx, dalloc(reinterpret_cast<RecordF64 **>(&pb.rec), ctx->cap));
       pb.cap = ctx->cap;
     }
-    if (!pb.arena_top) HIPCHK(ctx, dalloc(&pb.arena_top, 1));
+    if (!pb.arena_top) HIPCHK(ctx, dalloc(&pb.arena_top, kScanCounterWords));
     if (pb.tps_alloc < ctx->tps_alloc) {
       hipFree(pb.tps);
       pb.tps = nullptr;
@@ -964,7 +975,8 @@ int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_rb), sizeof(HostReadback), 0);
   if (e == hipSuccess) e = upload_atmosphere();
   if (e == hipSuccess) e = configure_tile_kernels();
-  if (e == hipSuccess) e = dalloc(&ctx->arena_top, 1);
+  if (e == hipSuccess) e = dalloc(&ctx->arena_top, kScanCounterWords);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
   if (e == hipSuccess) e = dalloc(&ctx->log_count, 1);
   if (e == hipSuccess) e = dalloc(&ctx->counters, 1);
   if (e == hipSuccess) e = dalloc(&ctx->n_alive, 1);
@@ -1432,15 +1444,16 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   ctx->tune_ens_plane = 0;
   ctx->tune_rows = 0;
   ctx->tune_bcells = 0;
+  ctx->tune_persistent = 0;
   int rc = setup_grid(ctx, n, rx.data());
   if (rc != LADDS_OK) return rc;
   const double base_cell = 1.0 / ctx->grid.inv_cell;
   const bool ensemble = ctx->tiles.nscen > 1;
   // current point of the search: {plane bits (-1: sized from the objects per plane), threads,
   // tile, chunks, workgroups per CU, dispatch order, ensemble objects per plane (0: 64), row
-  // height and bucket width in cells}
+  // height and bucket width in cells, persistent pair kernel}
   double cur_cell = base_cell, cur_fx = 1.0;
-  int cur[9] = {-1, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk, 0, 0, 0, 1, 1};
+  int cur[10] = {-1, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk, 0, 0, 0, 1, 1, 0};
   double default_us = 0.0;
   rc = time_detection(ctx, dt_interp, repetitions, &default_us);
   if (rc != LADDS_OK) return rc;
@@ -1458,12 +1471,13 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     ctx->tune_ens_plane = v[6];
     ctx->tune_rows = v[7];
     ctx->tune_bcells = v[8];
+    ctx->tune_persistent = v[9];
     return setup_grid(ctx, n, rx.data());
   };
   // Coordinate descent, one dimension at a time, keeping every improvement: chunks per
   // plane (load balance of dense planes), workgroup size, bucket shape, cell size, plane
-  // shape, buckets per plane, tile, LDS share (workgroups per CU), dispatch order, objects
-  // per plane (ensembles).  Ensembles take a second pass: their plane size, workgroup size
+  // shape, buckets per plane, tile, LDS share (workgroups per CU), dispatch order, persistent
+  // pair kernel, objects per plane (ensembles).  Ensembles take a second pass: their plane size, workgroup size
   // and buckets interact (measured on 256 scenarios: 256-object planes need 256 threads and
   // 512 buckets).
   const int chunk_opts[] = {1, 2, 4, 8, 16, 32};
@@ -1475,16 +1489,16 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   const int ens_opts[] = {32, 64, 128, 256, 512};
   struct Cand {
     double cell, fx;
-    std::array<int, 9> v;
+    std::array<int, 10> v;
   };
-  // dimensions in search order (9: bucket shape)
-  const int dims[] = {0, 1, 9, 2, 3, 4, 5, 6, 7, 8};
+  // dimensions in search order (9: bucket shape, 10: persistent pair kernel)
+  const int dims[] = {0, 1, 9, 2, 3, 4, 5, 6, 7, 10, 8};
   constexpr int kDims = (int)(sizeof(dims) / sizeof(dims[0]));
   for (int step = 0; step < (ensemble ? 2 * kDims : kDims); ++step) {
     const int dim = dims[step % kDims];
     if (dim == 8 && !ensemble) continue;
     std::vector<Cand> cands;
-    std::array<int, 9> c{cur[0], cur[1], cur[2], cur[3], cur[4], cur[5], cur[6], cur[7], cur[8]};
+    std::array<int, 10> c{cur[0], cur[1], cur[2], cur[3], cur[4], cur[5], cur[6], cur[7], cur[8], cur[9]};
     if (dim == 0)
       for (int k : chunk_opts) { c[3] = k; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 1)
@@ -1509,6 +1523,8 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
       for (int o = 0; o < 2; ++o) { c[5] = o; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 8)  // ensembles: objects per plane, buckets sized from it again
       for (int e : ens_opts) { c[6] = e; c[0] = -1; cands.push_back({cur_cell, cur_fx, c}); }
+    if (dim == 10)
+      for (int p = 0; p < 2; ++p) { c[9] = p; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 9)  // bucket shape in cells (y x z): 1 x 1, 2 x 1, 2 x 2
       for (int k = 0; k < 3; ++k) {
         c[7] = k ? 2 : 1;
@@ -1527,7 +1543,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
         best_us = us;
         cur_cell = cand.cell;
         cur_fx = cand.fx;
-        for (int k = 0; k < 9; ++k) cur[k] = cand.v[k];
+        for (int k = 0; k < 10; ++k) cur[k] = cand.v[k];
       }
     }
   }
@@ -1548,6 +1564,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     out->ensemble_plane_objects = (int32_t)ctx->tune_ens_plane;
     out->row_cells = ctx->tiles.half_rows ? 2 : 1;
     out->bucket_cells = ctx->tiles.half_buckets ? 2 : 1;
+    out->persistent = ctx->tiles.persistent_groups ? 1 : 0;
     out->scan_us = best_us;
     out->default_us = default_us;
     out->tried = tried;
@@ -1586,7 +1603,7 @@ int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *t) {
                   t->chunks <= 32 && t->workgroups_per_cu >= 0 && t->workgroups_per_cu <= 8 &&
                   (t->plane_order == 0 || t->plane_order == 1) && t->ensemble_plane_objects >= 0 &&
                   t->ensemble_plane_objects <= 65536 && t->row_cells >= 0 && t->row_cells <= 2 &&
-                  t->bucket_cells >= 0 && t->bucket_cells <= 2;
+                  t->bucket_cells >= 0 && t->bucket_cells <= 2 && (t->persistent == 0 || t->persistent == 1);
   if (!ok) return ctx->fail(LADDS_E_INVALID, "tuning values outside the tuner's range");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   ctx->tune_cell_km = t->cell_size_km;
@@ -1600,6 +1617,7 @@ int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *t) {
   ctx->tune_ens_plane = t->ensemble_plane_objects;
   ctx->tune_rows = t->row_cells;
   ctx->tune_bcells = t->bucket_cells;
+  ctx->tune_persistent = t->persistent;
   if (ctx->n == 0) return LADDS_OK;  // applied at the next ladds_set_objects
   std::vector<double> rx(ctx->n);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -81,7 +81,18 @@ struct TileParams {
   // 1: buckets are two cells wide along z (GridParams::inv_cell_z), and an object scans two
   // buckets of a row, chosen by the half of its bucket it lies in; 0: three buckets.
   uint32_t half_buckets;
+  // > 0: the pair kernel runs this many workgroups (as many as the CUs hold at once), each
+  // taking plane pairs from its XCD's queue until none is left; 0: one workgroup per plane
+  // pair.
+  uint32_t persistent_groups;
 };
+
+// Words of the pair scan's device counters: [0] the scratch arena's fill and, one 128-byte
+// line each, [kQueueStride * (1 + x)] the next run of XCD x's queue (persistent pair
+// kernel).  The partition kernel clears them.
+constexpr uint32_t kXcdCount = 8;
+constexpr uint32_t kQueueStride = 32;
+constexpr uint32_t kScanCounterWords = kQueueStride * (1 + kXcdCount);
 
 // Sorted record of the plain FP64 form and of the split form: the position the
 // object is binned by, and its index in the object arrays.

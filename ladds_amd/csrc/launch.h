@@ -45,8 +45,8 @@ hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, cons
                              const StepState *step_state, hipStream_t s);
 uint32_t pair_tiles_header_words(const TileParams &tp);
 // Pair-kernel workgroups of `threads` threads a CU holds by its registers alone (list
-// mode; 0 when the query fails).
-int pair_tiles_groups_per_cu(int form, uint32_t threads);
+// mode; the persistent instantiation when `persistent`; 0 when the query fails).
+int pair_tiles_groups_per_cu(int form, uint32_t threads, bool persistent = false);
 // Workgroup sizes of the pair kernel.
 inline bool valid_pair_threads(int t) { return t == 128 || t == 256 || t == 384 || t == 512 || t == 1024; }
 hipError_t configure_tile_kernels();

@@ -39,7 +39,7 @@ constexpr size_t kTpsOff = kRecOff + align256(sizeof(RecordF64) * kMaxObjects);
 constexpr size_t kArenaOff = kTpsOff + align256(sizeof(uint32_t) * (size_t)kMaxTiles * ((1u << kPlaneBitsX) + 1));
 constexpr size_t kArenaRecords = 2 * kMaxObjects;  // every record at most twice (own plane and halo)
 constexpr size_t kTopOff = kArenaOff + align256(sizeof(RecordF64) * kArenaRecords);
-constexpr size_t kCountersOff = kTopOff + 256;
+constexpr size_t kCountersOff = kTopOff + align256(sizeof(uint32_t) * kScanCounterWords);  // the pair scan's counter words
 constexpr size_t kScratchBytes = kCountersOff + align256(sizeof(ScanCounters));
 
 static_assert(sizeof(RecordF64) >= sizeof(RecordF32), "records");

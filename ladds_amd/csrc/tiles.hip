@@ -133,7 +133,8 @@ __global__ __launch_bounds__(kPartThreads) void k_tile_partition(const double *r
     end = min(n, base + tp.tile_size);
   }
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) cnt[p] = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *arena_top = 0;
+  // the pair scan's counters: arena fill and the persistent kernel's queue positions
+  if (blockIdx.x == 0 && threadIdx.x < 1 + kXcdCount) arena_top[threadIdx.x * kQueueStride] = 0;
   __syncthreads();
   // every thread owns kPartItems objects of the tile; their loads are issued together
   uint32_t key[kPartItems];
@@ -517,7 +518,7 @@ __device__ __forceinline__ void pair_plane(const PairTileArgs &a, uint32_t *lds,
 // kXcdRun consecutive plane pairs are given to one XCD (the runs still alternate over the
 // XCDs, which keeps their loads balanced), and most halo reads find the records the
 // neighbouring plane pair loaded in that L2.  Applies when G is a multiple of 8 kXcdRun.
-constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcds = kXcdCount;
 constexpr uint32_t kXcdRun = 8;
 __device__ __forceinline__ uint32_t xcd_local_item(uint32_t b, uint32_t g) {
   if (g % (kXcds * kXcdRun)) return b;
@@ -534,15 +535,48 @@ constexpr int kPairWavesPerEu = (FORM == kFormSplit || THREADS >= 512) ? 5 : 1;
 static_assert(sizeof(FatRecord) == kFatRecordBytes, "predicate record size");
 static_assert(sizeof(RecordF64) <= kArenaSlotBytes && sizeof(RecordF32) <= kArenaSlotBytes, "arena slot");
 
-// One workgroup per plane pair (grid nplanes * nscen * nchunk).
-template <int FORM, bool DIGEST, int THREADS, bool FAT>
+// PERSIST false: one workgroup per plane pair (grid nplanes * nscen * nchunk).
+// PERSIST true (tp.persistent_groups workgroups, as many as the CUs hold at once): each
+// workgroup loops over plane pairs.  Workgroup b (on XCD b mod 8, as workgroups are dealt to
+// the XCDs in turn) takes the next run r of its XCD's queue (a.arena_top[kQueueStride *
+// (1 + xcd)], cleared by the partition; one atomic per run), which is run r of the runs of 8
+// consecutive items that XCD owns in the one-workgroup-per-item mapping (xcd_local_item), and
+// scans its items, until the run starts past the last item.  Every wavefront of the
+// workgroup reads the same run, so all leave the loop together.  A separate instantiation:
+// the loop around the scan costs registers (96 VGPRs against 77) that the one-item kernel
+// keeps free.
+template <int FORM, bool DIGEST, int THREADS, bool FAT, bool PERSIST>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kPairWavesPerEu<FORM, THREADS>)))
 void k_pair_tiles(PairTileArgs a) {
   extern __shared__ uint32_t lds[];
   ScanAcc acc;
-  uint32_t item = xcd_local_item(blockIdx.x, gridDim.x);
-  if (a.tp.plane_order) item = a.tp.plane_order[item / a.tp.nchunk] * a.tp.nchunk + item % a.tp.nchunk;
-  pair_plane<FORM, DIGEST, THREADS, FAT>(a, lds, item, acc);
+  const uint32_t nchunk = a.tp.nchunk;
+  if constexpr (!PERSIST) {
+    uint32_t item = xcd_local_item(blockIdx.x, gridDim.x);
+    if (a.tp.plane_order) item = a.tp.plane_order[item / nchunk] * nchunk + item % nchunk;
+    pair_plane<FORM, DIGEST, THREADS, FAT>(a, lds, item, acc);
+  } else {
+    // the LDS header's last word, which pair_plane leaves alone (a static __shared__ word
+    // would push the kernel past the dynamic LDS limit it is configured with)
+    uint32_t &next_pos = lds[a.tp.lds_header_words - 1];
+    const uint32_t total = a.tp.nplanes * a.tp.nscen * nchunk;
+    const uint32_t xcd = blockIdx.x % kXcds;
+    uint32_t *queue = a.arena_top + kQueueStride * (1 + xcd);
+    for (;;) {
+      if (threadIdx.x == 0) next_pos = atomicAdd(queue, 1u);
+      __syncthreads();  // also: the previous item's LDS is no longer read
+      const uint32_t run = next_pos;
+      __syncthreads();  // every thread has read next_pos before thread 0 writes the next one
+      const uint32_t first = (run * kXcds + xcd) * kXcdRun;
+      if (first >= total) break;
+      for (uint32_t i = 0; i < kXcdRun && first + i < total; ++i) {
+        uint32_t item = first + i;
+        if (a.tp.plane_order) item = a.tp.plane_order[item / nchunk] * nchunk + item % nchunk;
+        pair_plane<FORM, DIGEST, THREADS, FAT>(a, lds, item, acc);
+        __syncthreads();  // the next item's set-up rewrites the LDS this one read
+      }
+    }
+  }
   flush_counters<DIGEST>(a.sink, acc);
 }
 
@@ -603,20 +637,26 @@ hipError_t launch_tile_partition(int form, const DeviceObjects &o, const uint32_
   return launch_tile_partition_as<RecordF32>(o, keys, tp, items, tps, arena_top, s, stage_bytes);
 }
 
+// The four (form, output) instantiations of one workgroup size, arena layout and mode.
+template <int THREADS, bool FAT, bool PERSIST>
+hipError_t set_pair_lds_limit_of(int max_lds) {
+  const void *fs[] = {reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS, FAT, PERSIST>),
+                      reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, true, THREADS, FAT, PERSIST>),
+                      reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS, FAT, PERSIST>),
+                      reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, true, THREADS, FAT, PERSIST>)};
+  for (const void *f : fs) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 template <int THREADS>
 hipError_t set_pair_lds_limit(int max_lds) {
-  hipError_t e = hipSuccess;
-  auto set = [&](const void *f) {
-    if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-  };
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS, false>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, true, THREADS, false>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS, false>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, true, THREADS, false>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS, true>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, true, THREADS, true>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS, true>));
-  set(reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, true, THREADS, true>));
+  hipError_t e = set_pair_lds_limit_of<THREADS, false, false>(max_lds);
+  if (e == hipSuccess) e = set_pair_lds_limit_of<THREADS, true, false>(max_lds);
+  if (e == hipSuccess) e = set_pair_lds_limit_of<THREADS, false, true>(max_lds);
+  if (e == hipSuccess) e = set_pair_lds_limit_of<THREADS, true, true>(max_lds);
   return e;
 }
 
@@ -639,37 +679,54 @@ hipError_t configure_tile_kernels() {
   return e;
 }
 
-template <int THREADS>
+template <int THREADS, bool PERSIST>
 int pair_groups_per_cu_as(int form) {
   int n = 0;
-  const void *f = form == kFormFp64 ? reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS, false>)
-                                    : reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS, false>);
+  const void *f = form == kFormFp64
+                      ? reinterpret_cast<const void *>(&k_pair_tiles<kFormFp64, false, THREADS, false, PERSIST>)
+                      : reinterpret_cast<const void *>(&k_pair_tiles<kFormSplit, false, THREADS, false, PERSIST>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, THREADS, 0) != hipSuccess) return 0;
   return n;
 }
 
-int pair_tiles_groups_per_cu(int form, uint32_t threads) {
+template <bool PERSIST>
+int pair_groups_per_cu_mode(int form, uint32_t threads) {
   switch (threads) {
     case 128:
-      return pair_groups_per_cu_as<128>(form);
+      return pair_groups_per_cu_as<128, PERSIST>(form);
     case 256:
-      return pair_groups_per_cu_as<256>(form);
+      return pair_groups_per_cu_as<256, PERSIST>(form);
     case 384:
-      return pair_groups_per_cu_as<384>(form);
+      return pair_groups_per_cu_as<384, PERSIST>(form);
     case 512:
-      return pair_groups_per_cu_as<512>(form);
+      return pair_groups_per_cu_as<512, PERSIST>(form);
     case 1024:
-      return pair_groups_per_cu_as<1024>(form);
+      return pair_groups_per_cu_as<1024, PERSIST>(form);
     default:
       return 0;
   }
 }
 
+int pair_tiles_groups_per_cu(int form, uint32_t threads, bool persistent) {
+  return persistent ? pair_groups_per_cu_mode<true>(form, threads) : pair_groups_per_cu_mode<false>(form, threads);
+}
+
 uint32_t pair_tiles_header_words(const TileParams &tp) {
   const uint32_t L = 1u << tp.plane_bits;
   const uint32_t T = tp.nscen > 1 ? tp.max_scen_tiles : tp.ntiles;
-  uint32_t w = 4 * T + 2 + 2 * L + 2 + kMaxPairThreads / 64 + 1 + 4;
+  // segment tables, bucket cursors, wavefront sums, 4 scratch words, and a last word that
+  // only the persistent kernel uses (its queue position)
+  uint32_t w = 4 * T + 2 + 2 * L + 2 + kMaxPairThreads / 64 + 1 + 4 + 1;
   return (w + 7u) & ~7u;  // 32-byte alignment of the record array
+}
+
+template <int THREADS, bool FAT, bool PERSIST>
+void launch_pair_variant(bool f64, bool digest, dim3 grid, dim3 block, size_t shm, hipStream_t s,
+                         const PairTileArgs &a) {
+  if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false, THREADS, FAT, PERSIST>), grid, block, shm, s, a);
+  if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true, THREADS, FAT, PERSIST>), grid, block, shm, s, a);
+  if (!f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, false, THREADS, FAT, PERSIST>), grid, block, shm, s, a);
+  if (!f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, true, THREADS, FAT, PERSIST>), grid, block, shm, s, a);
 }
 
 template <int THREADS>
@@ -677,19 +734,12 @@ void launch_pair_tiles_as(int form, bool digest, const PairTileArgs &a, hipStrea
   const TileParams &tp = a.tp;
   const size_t rec = (form == kFormFp64) ? sizeof(RecordF64) : sizeof(RecordF32);
   const size_t shm = sizeof(uint32_t) * tp.lds_header_words + rec * tp.cap_items;
-  const dim3 grid(tp.nplanes * tp.nscen * tp.nchunk), block(THREADS);
-  const bool f64 = (form == kFormFp64);
-  if (tp.arena_fat) {
-    if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false, THREADS, true>), grid, block, shm, s, a);
-    if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true, THREADS, true>), grid, block, shm, s, a);
-    if (!f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, false, THREADS, true>), grid, block, shm, s, a);
-    if (!f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, true, THREADS, true>), grid, block, shm, s, a);
-  } else {
-    if (f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, false, THREADS, false>), grid, block, shm, s, a);
-    if (f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormFp64, true, THREADS, false>), grid, block, shm, s, a);
-    if (!f64 && !digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, false, THREADS, false>), grid, block, shm, s, a);
-    if (!f64 && digest) hipLaunchKernelGGL((k_pair_tiles<kFormSplit, true, THREADS, false>), grid, block, shm, s, a);
-  }
+  const dim3 grid(tp.persistent_groups ? tp.persistent_groups : tp.nplanes * tp.nscen * tp.nchunk), block(THREADS);
+  const bool f64 = (form == kFormFp64), persist = tp.persistent_groups != 0u;
+  if (tp.arena_fat && persist) launch_pair_variant<THREADS, true, true>(f64, digest, grid, block, shm, s, a);
+  if (tp.arena_fat && !persist) launch_pair_variant<THREADS, true, false>(f64, digest, grid, block, shm, s, a);
+  if (!tp.arena_fat && persist) launch_pair_variant<THREADS, false, true>(f64, digest, grid, block, shm, s, a);
+  if (!tp.arena_fat && !persist) launch_pair_variant<THREADS, false, false>(f64, digest, grid, block, shm, s, a);
 }
 
 hipError_t launch_pair_tiles(int form, bool digest, const DeviceObjects &o, const GridParams &g,

@@ -94,8 +94,9 @@ def test_simulate_compacts_only_after_burnups(graph):
 
 def test_set_tuning_applies_a_saved_shape():
     """ladds_set_tuning: the shape one context's tuner chose, changed in workgroup size, tile,
-    dispatch order and bucket shape and applied to a second context without timing, gives
-    the same pair set (and the oracle's); values outside the tuner's range are refused."""
+    dispatch order, bucket shape and pair-kernel mode and applied to a second context without
+    timing, gives the same pair set (and the oracle's); values outside the tuner's range are
+    refused."""
     pop = la.generate_population(1, 0x1ADD5 + 14, 65536, planted_pairs=256)
     with _engine(precision=la.SPLIT) as eng:
         eng.set_objects(pop)
@@ -103,7 +104,7 @@ def test_set_tuning_applies_a_saved_shape():
         hits_tuned, _ = eng.conjunctions()
     t2 = dict(t, pair_threads=512 if t["pair_threads"] != 512 else 384, tile_size=4096,
               plane_order=1 - t["plane_order"], row_cells=1 if t["row_cells"] == 2 else 2,
-              bucket_cells=1 if t["bucket_cells"] == 2 else 2)
+              bucket_cells=1 if t["bucket_cells"] == 2 else 2, persistent=1 - t["persistent"])
     with _engine(precision=la.SPLIT) as eng:
         eng.set_objects(pop)
         eng.set_tuning(t2)
@@ -118,6 +119,8 @@ def test_set_tuning_applies_a_saved_shape():
             eng.set_tuning(dict(t2, row_cells=3))
         with pytest.raises(la.LaddsError):
             eng.set_tuning(dict(t2, bucket_cells=-1))
+        with pytest.raises(la.LaddsError):
+            eng.set_tuning(dict(t2, persistent=2))
     oh, _ = ol.pairs(ol.ObjectSet(pop.r(), pop.v(), pop.radius_m, pop.activity, pop.id, pop.parent),
                      ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0))
     key = lambda p: sorted(zip(p["id_a"].tolist(), p["id_b"].tolist()))

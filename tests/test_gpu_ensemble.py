@@ -118,7 +118,14 @@ def test_bad_offsets_are_rejected():
             eng.set_scenarios(bad)
 
 
-def _run(pop, off, graph, steps, rounds):
+# a plane-tile shape with the persistent pair kernel (workgroups taking plane pairs from
+# per-XCD queues), valid for the split form's 10 km cutoff
+PERSISTENT_SHAPE = {"cell_size_km": 12.625125012625125, "plane_width_km": 25.25025002525025, "plane_bits": 9,
+                    "pair_threads": 128, "tile_size": 4096, "chunks": 1, "workgroups_per_cu": 0, "plane_order": 1,
+                    "ensemble_plane_objects": 128, "row_cells": 2, "bucket_cells": 2, "persistent": 1}
+
+
+def _run(pop, off, graph, steps, rounds, shape=None):
     cfg = dict(dt=10.0, components=la.ALL_COMPONENTS, precision=la.SPLIT, cutoff_km=10.0,
                threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, step_graph=graph)
     out = []
@@ -126,6 +133,8 @@ def _run(pop, off, graph, steps, rounds):
         eng.set_objects(pop)
         if off is not None:
             eng.set_scenarios(off)
+        if shape is not None:
+            eng.set_tuning(shape)
         for k in range(rounds):
             st = eng.simulate(steps, first_iteration=k * steps, tpcd=1, min_altitude_km=150.0)
             log = eng.fetch_log()
@@ -139,17 +148,21 @@ def _run(pop, off, graph, steps, rounds):
     return out
 
 
+@pytest.mark.parametrize("persistent", [False, True])
 @pytest.mark.parametrize("ensemble", [False, True])
-def test_graph_replay_matches_eager_launches(ensemble):
+def test_graph_replay_matches_eager_launches(ensemble, persistent):
     """Each simulate() call captures one step graph and replays it per step; three calls in a
     row, with eager conjunction scans and synchronisations between them, must reproduce the
-    eagerly launched run exactly (pair logs with iterations, states, counters)."""
+    eagerly launched run exactly (pair logs with iterations, states, counters), with one pair
+    workgroup per plane pair and with the persistent pair kernel (its queues are cleared by
+    the partition inside the graph, never by a memset)."""
     if ensemble:
         pop, off = _ensemble()
     else:
         pop, off = la.generate_population(1, 0x1ADD5 + 21, 20000, planted_pairs=64), None
-    eager = _run(pop, off, 0, 30, 3)
-    graph = _run(pop, off, 1, 30, 3)
+    shape = PERSISTENT_SHAPE if persistent else None
+    eager = _run(pop, off, 0, 30, 3, shape)
+    graph = _run(pop, off, 1, 30, 3, shape)
     for a, b in zip(eager, graph):
         assert a[:3] == b[:3]
         assert np.array_equal(a[3], b[3])
@@ -162,8 +175,8 @@ def test_graph_replay_matches_eager_launches(ensemble):
 def test_ensemble_plane_sizes_keep_pairs(plane_objects):
     """Ensemble planes sized for more (or fewer) objects fold more (or fewer) distant x-slabs
     into one plane of a scenario, in plane order and in the heaviest-first dispatch order
-    over all scenarios' planes, with buckets of one or two cells; each scenario's pair set
-    still equals the oracle's."""
+    over all scenarios' planes, with buckets of one or two cells, one workgroup per plane pair
+    or persistent workgroups; each scenario's pair set still equals the oracle's."""
     pop, off = _ensemble()
     cfg = dict(dt=10.0, components=la.KEP | la.J2, precision=la.SPLIT, cutoff_km=10.0,
                threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0)
@@ -172,7 +185,8 @@ def test_ensemble_plane_sizes_keep_pairs(plane_objects):
         eng.set_scenarios(off)
         t = eng.autotune(repetitions=1)
         eng.set_tuning(dict(t, ensemble_plane_objects=plane_objects, plane_order=int(plane_objects >= 256),
-                            row_cells=2 if plane_objects != 256 else 1, bucket_cells=2 if plane_objects == 32 else 1))
+                            row_cells=2 if plane_objects != 256 else 1, bucket_cells=2 if plane_objects == 32 else 1,
+                            persistent=int(plane_objects != 256)))
         eng.propagate(2)
         hits, ev = eng.conjunctions()
         r, v = eng.state()

@@ -544,6 +544,36 @@ def test_two_cell_buckets_keep_the_pair_set(rows, width, plane_bits, chunks, pre
     assert ecnt == oe.shape[0] and edig == ol.digest(oe)
 
 
+@pytest.mark.parametrize("threads,plane_bits,chunks,cells", [(256, 10, 1, 2), (128, 4, 1, 1), (384, 7, 4, 2),
+                                                                (1024, 10, 3, 1)])
+@pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
+def test_persistent_pair_kernel_keeps_the_pair_set(threads, plane_bits, chunks, cells, precision):
+    """The persistent pair kernel (as many workgroups as the CUs hold, each taking plane pairs
+    from its XCD's queue until none is left) over one workgroup size per case, small and
+    large bucket counts, chunked planes and the cluster's plane pair sorted in the global
+    arena: pair list and digest equal the oracle's, twice in a row on one context (the
+    partition clears the queues for each scan)."""
+    pop = _cluster_population(20)
+    cell = 10.11 if precision == la.SPLIT else 10.0 * (1 + 1e-8)  # >= the search radius
+    shape = {"cell_size_km": cell, "plane_width_km": cell, "plane_bits": plane_bits, "pair_threads": threads,
+             "tile_size": 8192, "chunks": chunks, "workgroups_per_cu": 0, "plane_order": 1, "row_cells": cells,
+             "bucket_cells": cells, "persistent": 1}
+    oh, oe = ol.pairs(_oracle_objects(pop.r(), pop.v(), pop),
+                      ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0))
+    assert oh.shape[0] > 1000
+    with engine(dt=10.0, precision=precision, cutoff_km=10.0, threshold_mode=la.THRESHOLD_ABSOLUTE,
+                threshold_km=1.0) as eng:
+        eng.set_objects(pop)
+        eng.set_tuning(shape)
+        for _ in range(2):
+            hits, ev = eng.conjunctions()
+            cnt, dig, ecnt, edig = eng.conjunction_digest()
+            _pair_equal(ol.sort_pairs(hits), oh)
+            _pair_equal(ol.sort_pairs(ev), oe)
+            assert cnt == oh.shape[0] and dig == ol.digest(oh)
+            assert ecnt == oe.shape[0] and edig == ol.digest(oe)
+
+
 @pytest.mark.parametrize("threads", [128, 256, 384, 512, 1024])
 @pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
 def test_every_workgroup_size_keeps_the_pair_set(threads, precision):
