@@ -130,8 +130,10 @@ def calibrate_copy(mib):
 
 
 def make_engine(la, precision):
+    # the tuner times each shape over simulate() steps (the loop this bench times)
     cfg = la.EngineConfig(dt=10.0, components=la.ALL_COMPONENTS, precision=precision, cutoff_km=10.0,
-                          threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, log_capacity=1 << 24)
+                          threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, log_capacity=1 << 24,
+                          tune_simulate=1)
     return la.Engine(cfg)
 
 
@@ -302,7 +304,8 @@ def ensemble_leg(la, steps, warmup=10):
            "objects": int(pop.n), "steps": steps}
     res = {}
     cfg = la.EngineConfig(dt=10.0, components=la.ALL_COMPONENTS, precision=la.SPLIT, cutoff_km=10.0,
-                          threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, log_capacity=1 << 22)
+                          threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, log_capacity=1 << 22,
+                          tune_simulate=1)
     with la.Engine(cfg) as eng:
         eng.set_objects(pop)
         eng.set_scenarios(off)
@@ -372,7 +375,7 @@ def clustered_leg(la, steps, cpu_check=True, pinned=None):
     for prec, name in ((la.FP64, "fp64"), (la.SPLIT, "split")):
         cfg = la.EngineConfig(dt=10.0, components=la.ALL_COMPONENTS, precision=prec, cutoff_km=5.0,
                               threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=0.5, pair_output=la.PAIRS_DIGEST,
-                              log_capacity=1 << 16)
+                              log_capacity=1 << 16)  # a step takes ~75 ms: tune on single detections
         with la.Engine(cfg) as eng:
             eng.set_objects(pop)
             if pinned:

@@ -106,6 +106,9 @@ typedef struct ladds_config {
   int32_t step_graph;                /* ladds_simulate with tpcd 1: replay each step from one  */
                                      /* captured HIP graph (plane-tile scan, no profiling)     */
   int32_t scan_chunks;               /* plane-tile scan: most workgroups per plane, 1-32 (0: auto) */
+  int32_t tune_simulate;             /* ladds_autotune times each shape over 48 ladds_simulate    */
+                                     /* steps (state, time, pair log and counters restored after), */
+                                     /* as AutoPas times real iterations; 0: one detection alone   */
 } ladds_config;
 
 /* Object arrays, structure-of-arrays, host memory; copied in (caller keeps ownership). */
@@ -281,6 +284,7 @@ typedef struct ladds_tuning {
   int32_t plane_bits;    /* chosen log2 buckets per x-plane                     */
   int32_t pair_threads;  /* chosen pair-kernel workgroup size                   */
   double scan_us;        /* device time of one detection with the choice [us]   */
+                         /* (ladds_config.tune_simulate: wall time per step)    */
   double default_us;     /* the same for the untuned default [us]               */
   int32_t tried;         /* configurations timed                                */
   int32_t tile_size;     /* chosen objects per partition tile                   */

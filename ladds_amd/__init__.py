@@ -79,6 +79,7 @@ class EngineConfig:
     log_capacity: int = 1 << 20
     step_graph: int = 0                   # replay each simulate() step from a captured HIP graph
     scan_chunks: int = 0                  # workgroups per plane of the plane-tile scan (0: auto)
+    tune_simulate: int = 0                # autotune() times each shape over simulate() steps
 
     def to_c(self) -> _abi.Config:
         c = _abi.Config()

@@ -49,6 +49,7 @@ class Config(C.Structure):
         ("log_capacity", C.c_uint64),
         ("step_graph", C.c_int32),
         ("scan_chunks", C.c_int32),
+        ("tune_simulate", C.c_int32),
     ]
 
 

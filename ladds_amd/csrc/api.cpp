@@ -945,6 +945,7 @@ void ladds_config_default(ladds_config *cfg) {
   cfg->log_capacity = 1u << 20;
   cfg->step_graph = 0;
   cfg->scan_chunks = 0;
+  cfg->tune_simulate = 0;
 }
 
 int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
@@ -1399,6 +1400,11 @@ static int run_detection_sync(ladds_ctx *ctx, double dt_interp, bool digest, Sca
 
 // Device time of one detection (keys, partition, pair scan in digest mode) on the
 // current state: the minimum over `reps` runs after one warm-up run.
+// Smallest relative gain for which the autotuner moves to a candidate shape.
+constexpr double kTuneMargin = 0.01;
+
+// Device time of one detection (partition + pair scan, digest output) with the current
+// shape: the best of `reps` runs after a warm-up.
 static int time_detection(ladds_ctx *ctx, double dt_interp, int reps, double *us) {
   hipEvent_t e0, e1;
   HIPCHK(ctx, hipEventCreate(&e0));
@@ -1417,6 +1423,106 @@ static int time_detection(ladds_ctx *ctx, double dt_interp, int reps, double *us
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   *us = best;
+  return rc;
+}
+
+// Steps of one timed ladds_simulate burst (ladds_config.tune_simulate): long enough that the
+// call's fixed cost and the timing noise stay below the tuner's 1% margin.
+constexpr int kTuneSteps = 48;
+
+// What a ladds_simulate burst changes, kept aside by the tuner (ladds_config.tune_simulate):
+// the state, activity and parent ids (spawn protection may reset them) of the n objects, and
+// the host-side step count and spawn-protection count.
+struct SimSnapshot {
+  double *r[3] = {nullptr, nullptr, nullptr}, *v[3] = {nullptr, nullptr, nullptr};
+  int32_t *act = nullptr;
+  uint64_t *parent = nullptr;
+  uint64_t step_count = 0, quiet = 0;
+  bool protected_objects = false;
+  void release() {
+    for (int k = 0; k < 3; ++k) {
+      hipFree(r[k]);
+      hipFree(v[k]);
+    }
+    hipFree(act);
+    hipFree(parent);
+  }
+};
+
+int snapshot_copy(ladds_ctx *ctx, SimSnapshot &s, bool save) {
+  const size_t n = ctx->n;
+  double *dr[3] = {ctx->d.rx, ctx->d.ry, ctx->d.rz}, *dv[3] = {ctx->d.vx, ctx->d.vy, ctx->d.vz};
+  auto cp = [&](void *live, void *kept, size_t bytes) {
+    return hipMemcpyAsync(save ? kept : live, save ? live : kept, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+  };
+  for (int k = 0; k < 3; ++k) {
+    HIPCHK(ctx, cp(dr[k], s.r[k], sizeof(double) * n));
+    HIPCHK(ctx, cp(dv[k], s.v[k], sizeof(double) * n));
+  }
+  HIPCHK(ctx, cp(ctx->d.activity, s.act, sizeof(int32_t) * n));
+  HIPCHK(ctx, cp(ctx->d.parent, s.parent, sizeof(uint64_t) * n));
+  if (save) {
+    s.step_count = ctx->step_count;
+    s.quiet = ctx->quiet;
+    s.protected_objects = ctx->protected_objects;
+  } else {
+    ctx->step_count = s.step_count;
+    ctx->quiet = s.quiet;
+    ctx->protected_objects = s.protected_objects;
+    ctx->keys_valid = false;
+    ctx->env_len = 0;
+  }
+  return LADDS_OK;
+}
+
+// Device time per step of kTuneSteps ladds_simulate steps (detection every step, no
+// burn-up; events on the context's stream around the call, which returns synchronised) with
+// the current shape: the best of `reps` bursts after a warm-up, each from the saved state,
+// which is restored at the end with the time, the spawn-protection count and an empty pair
+// log.
+int time_simulate(ladds_ctx *ctx, int reps, double *us) {
+  SimSnapshot snap;
+  const size_t n = ctx->n;
+  int rc = LADDS_OK;
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+    e = dalloc(&snap.r[k], n);
+    if (e == hipSuccess) e = dalloc(&snap.v[k], n);
+  }
+  if (e == hipSuccess) e = dalloc(&snap.act, n);
+  if (e == hipSuccess) e = dalloc(&snap.parent, n);
+  if (e != hipSuccess) {
+    snap.release();
+    return ctx->hipfail(e, "tuner snapshot allocation");
+  }
+  rc = snapshot_copy(ctx, snap, true);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == LADDS_OK && ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess))
+    rc = ctx->hipfail(e, "tuner events");
+  double best = 1e30;
+  for (int r = 0; r <= reps && rc == LADDS_OK; ++r) {
+    (void)hipEventRecord(e0, ctx->stream);
+    rc = ladds_simulate(ctx, 0, kTuneSteps, 1, 0.0, nullptr);
+    (void)hipEventRecord(e1, ctx->stream);
+    float ms = 0.f;
+    if (rc == LADDS_OK && (e = hipEventSynchronize(e1)) == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (rc == LADDS_OK && e != hipSuccess) rc = ctx->hipfail(e, "tuner timing");
+    if (rc == LADDS_OK) rc = snapshot_copy(ctx, snap, false);
+    if (rc == LADDS_OK) {
+      e = hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream);
+      if (e != hipSuccess) rc = ctx->hipfail(e, "tuner log reset");
+      ctx->log_nonempty = false;
+    }
+    if (rc == LADDS_OK && r > 0) best = std::min(best, 1000.0 * ms);
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (rc == LADDS_OK) {
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = ctx->hipfail(e, "tuner restore");
+  }
+  snap.release();
+  *us = best / kTuneSteps;
   return rc;
 }
 
@@ -1455,7 +1561,12 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   double cur_cell = base_cell, cur_fx = 1.0;
   int cur[10] = {-1, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk, 0, 0, 0, 1, 1, 0};
   double default_us = 0.0;
-  rc = time_detection(ctx, dt_interp, repetitions, &default_us);
+  // the objective: one detection, or (ladds_config.tune_simulate) a burst of simulate steps
+  auto measure = [&](double *t) {
+    return ctx->cfg.tune_simulate ? time_simulate(ctx, repetitions, t)
+                                  : time_detection(ctx, dt_interp, repetitions, t);
+  };
+  rc = measure(&default_us);
   if (rc != LADDS_OK) return rc;
   double best_us = default_us;
   int tried = 1;
@@ -1536,10 +1647,13 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
       rc = apply(cand.cell, cand.fx, cand.v.data());
       if (rc != LADDS_OK) return rc;
       double us = 0.0;
-      rc = time_detection(ctx, dt_interp, repetitions, &us);
+      rc = measure(&us);
       if (rc != LADDS_OK) return rc;
       ++tried;
-      if (us < best_us) {
+      // keep a candidate only when it is faster by more than the timing noise: near-ties
+      // (the 256-scenario ensemble has shapes within 0.2% in detection time that differ by 6%
+      // in the overlapped loop) then stay with the earlier, default-side choice
+      if (us < best_us * (1.0 - kTuneMargin)) {
         best_us = us;
         cur_cell = cand.cell;
         cur_fx = cand.fx;

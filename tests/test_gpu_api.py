@@ -92,6 +92,37 @@ def test_simulate_compacts_only_after_burnups(graph):
         assert st["remaining"] == 5000 - low.size == eng.n
 
 
+def test_simulate_tuning_restores_the_context():
+    """ladds_config.tune_simulate: the tuner times every shape over a burst of simulate steps
+    run from the current state.  Afterwards the state, the time (later steps see the same
+    Sun/Moon positions), the pair log (empty) and the spawn-protection count are those from
+    before, so the run continues bit for bit as on a context tuned on detections alone."""
+    pop = la.generate_population(1, 0x1ADD5 + 15, 65536, planted_pairs=256)
+    pop.parent[:64] = 7  # protected objects: the bursts run the spawn-protection bookkeeping
+    runs = []
+    for tune_simulate in (0, 1):
+        cfg = la.EngineConfig(dt=10.0, components=la.ALL_COMPONENTS, precision=la.SPLIT, cutoff_km=10.0,
+                              threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, tune_simulate=tune_simulate)
+        with la.Engine(cfg) as eng:
+            eng.set_objects(pop)
+            eng.propagate(2)
+            r0, v0 = eng.state()
+            t = eng.autotune(repetitions=1)
+            r1, v1 = eng.state()
+            assert np.array_equal(r0, r1) and np.array_equal(v0, v1)
+            assert t["tried"] > 10
+            assert eng.fetch_log().shape[0] == 0
+            hits, _ = eng.conjunctions()
+            eng.simulate(5, first_iteration=2, tpcd=1)
+            log = eng.fetch_log()
+            r2, v2 = eng.state()
+        runs.append((sorted(zip(hits["id_a"].tolist(), hits["id_b"].tolist())),
+                     sorted((int(e["iteration"]), int(e["pair"]["id_a"]), int(e["pair"]["id_b"])) for e in log), r2, v2))
+    assert len(runs[0][0]) > 0 and len(runs[0][1]) > 0
+    assert runs[0][0] == runs[1][0] and runs[0][1] == runs[1][1]
+    assert np.array_equal(runs[0][2], runs[1][2]) and np.array_equal(runs[0][3], runs[1][3])
+
+
 def test_set_tuning_applies_a_saved_shape():
     """ladds_set_tuning: the shape one context's tuner chose, changed in workgroup size, tile,
     dispatch order, bucket shape and pair-kernel mode and applied to a second context without
