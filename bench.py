@@ -304,8 +304,7 @@ def ensemble_leg(la, steps, warmup=10):
            "objects": int(pop.n), "steps": steps}
     res = {}
     cfg = la.EngineConfig(dt=10.0, components=la.ALL_COMPONENTS, precision=la.SPLIT, cutoff_km=10.0,
-                          threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, log_capacity=1 << 22,
-                          tune_simulate=1)
+                          threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, log_capacity=1 << 22)
     with la.Engine(cfg) as eng:
         eng.set_objects(pop)
         eng.set_scenarios(off)

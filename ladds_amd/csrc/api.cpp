@@ -1602,11 +1602,17 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     double cell, fx;
     std::array<int, 10> v;
   };
-  // dimensions in search order (9: bucket shape, 10: persistent pair kernel)
+  // dimensions in search order (9: bucket shape, 10: persistent pair kernel), one pass (two
+  // for ensembles), then 11: cell size and buckets per plane together -- one at a time, a
+  // cell size that ties at the current bucket count is kept although it loses with more
+  // buckets (measured on the 256-scenario ensemble: 12.6 km cells with 512 buckets per
+  // plane 6% faster than the 10.1 or 15.2 km cells with 128 that the passes stopped at)
   const int dims[] = {0, 1, 9, 2, 3, 4, 5, 6, 7, 10, 8};
   constexpr int kDims = (int)(sizeof(dims) / sizeof(dims[0]));
-  for (int step = 0; step < (ensemble ? 2 * kDims : kDims); ++step) {
-    const int dim = dims[step % kDims];
+  std::vector<int> order(dims, dims + kDims);
+  if (ensemble) order.insert(order.end(), dims, dims + kDims);
+  order.push_back(11);
+  for (const int dim : order) {
     if (dim == 8 && !ensemble) continue;
     std::vector<Cand> cands;
     std::array<int, 10> c{cur[0], cur[1], cur[2], cur[3], cur[4], cur[5], cur[6], cur[7], cur[8], cur[9]};
@@ -1614,10 +1620,21 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
       for (int k : chunk_opts) { c[3] = k; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 1)
       for (int t : thread_opts) { c[1] = t; cands.push_back({cur_cell, cur_fx, c}); }
+    // cell size and plane shape change the objects per bucket: each option is tried with the
+    // current buckets per plane and with the count sized from it again (they interact: on the
+    // 256-scenario ensemble a fixed count kept the search in a 6% slower basin)
+    std::array<int, 10> c_auto = c;
+    c_auto[0] = -1;
     if (dim == 2)
-      for (double f : cell_opts) cands.push_back({base_cell * f, cur_fx, c});
+      for (double f : cell_opts) {
+        cands.push_back({base_cell * f, cur_fx, c});
+        if (c[0] != -1) cands.push_back({base_cell * f, cur_fx, c_auto});
+      }
     if (dim == 3)
-      for (double f : fx_opts) cands.push_back({cur_cell, f, c});
+      for (double f : fx_opts) {
+        cands.push_back({cur_cell, f, c});
+        if (c[0] != -1) cands.push_back({cur_cell, f, c_auto});
+      }
     if (dim == 4) {
       // buckets per plane around the count the current plane size gives
       rc = apply(cur_cell, cur_fx, cur);
@@ -1636,6 +1653,17 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
       for (int e : ens_opts) { c[6] = e; c[0] = -1; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 10)
       for (int p = 0; p < 2; ++p) { c[9] = p; cands.push_back({cur_cell, cur_fx, c}); }
+    if (dim == 11) {
+      rc = apply(cur_cell, cur_fx, cur);
+      if (rc != LADDS_OK) return rc;
+      const int pb_now = (int)(ctx->grid.by + ctx->grid.bz);
+      for (double f : cell_opts)
+        for (int d = -1; d <= 2; ++d)
+          if (pb_now + d >= 4 && pb_now + d <= 12) {
+            c[0] = pb_now + d;
+            cands.push_back({base_cell * f, cur_fx, c});
+          }
+    }
     if (dim == 9)  // bucket shape in cells (y x z): 1 x 1, 2 x 1, 2 x 2
       for (int k = 0; k < 3; ++k) {
         c[7] = k ? 2 : 1;
