@@ -1496,6 +1496,7 @@ int time_simulate(ladds_ctx *ctx, int reps, double *us) {
     return ctx->hipfail(e, "tuner snapshot allocation");
   }
   rc = snapshot_copy(ctx, snap, true);
+  const bool saved = rc == LADDS_OK;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (rc == LADDS_OK && ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess))
     rc = ctx->hipfail(e, "tuner events");
@@ -1517,10 +1518,17 @@ int time_simulate(ladds_ctx *ctx, int reps, double *us) {
   }
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
-  if (rc == LADDS_OK) {
-    e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) rc = ctx->hipfail(e, "tuner restore");
+  if (saved && rc != LADDS_OK) {
+    // a failed burst: the context still continues from the saved state with an empty log
+    // (the first error is the one reported)
+    (void)hipStreamSynchronize(ctx->stream2);
+    (void)hipStreamSynchronize(ctx->stream3);
+    (void)snapshot_copy(ctx, snap, false);
+    (void)hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream);
+    ctx->log_nonempty = false;
   }
+  e = hipStreamSynchronize(ctx->stream);
+  if (rc == LADDS_OK && e != hipSuccess) rc = ctx->hipfail(e, "tuner restore");
   snap.release();
   *us = best / kTuneSteps;
   return rc;
