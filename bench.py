@@ -515,6 +515,11 @@ def main():
         out = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_us": k["avg_us"],
                "bytes_per_object": per_obj}
+        if name == "pair_tiles":
+            # its bytes are far below what HBM allows: the limit is elsewhere
+            out["note"] = ("bound by instruction issue and LDS latency in the neighbour scan, not HBM: "
+                           "waves wait ~55% of their lifetime, VALU busy about half the kernel "
+                           "(profiles/pair_kernel_counters_1M.md)")
         flops = MEASURED_FLOPS_PER_OBJECT.get((name, form))
         if flops:
             f64 = flops[0] * a.objects / (k["avg_us"] * 1e-6) / 1e12
