@@ -91,17 +91,25 @@ def test_simulate_logs_scenarios_and_compaction_keeps_them():
     assert all((int(h["id_a"]) >> 32) == (int(h["id_b"]) >> 32) for h in hits)
 
 
-def test_autotune_with_scenarios_keeps_pairs():
+@pytest.mark.parametrize("tune_simulate", [0, 1])
+def test_autotune_with_scenarios_keeps_pairs(tune_simulate):
+    """Autotuning an ensemble, on single detections or on simulate bursts (which run over
+    every scenario and are undone): the pair set, the state and the scenario offsets after it
+    are those before."""
     pop, off = _ensemble()
     cfg = dict(dt=10.0, components=la.KEP, precision=la.SPLIT, cutoff_km=10.0,
-               threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0)
+               threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, tune_simulate=tune_simulate)
     with la.Engine(la.EngineConfig(**cfg)) as eng:
         eng.set_objects(pop)
         eng.set_scenarios(off)
         before = eng.conjunctions()[0]
+        r0, v0 = eng.state()
         eng.autotune(repetitions=1)
         after = eng.conjunctions()[0]
+        r1, v1 = eng.state()
+        assert np.array_equal(eng.scenarios(), off)
     assert _key(ol.sort_pairs(before)) == _key(ol.sort_pairs(after))
+    assert np.array_equal(r0, r1) and np.array_equal(v0, v1)
 
 
 def test_bad_offsets_are_rejected():
