@@ -106,6 +106,8 @@ def parse():
     p.add_argument("--tuning", metavar="JSON",
                    help="apply these plane-tile shapes instead of autotuning: a JSON object per form, as the "
                         "'tuning' field of an earlier run's line (profiler runs then measure the tuned shape)")
+    p.add_argument("--ensemble-tuning", metavar="JSON",
+                   help="apply this plane-tile shape (one JSON object) in the ensemble leg instead of autotuning it")
     p.add_argument("--clustered-tuning", metavar="JSON",
                    help="apply this plane-tile shape (one JSON object, both forms) in the clustered leg instead "
                         "of autotuning it")
@@ -291,7 +293,7 @@ def _pair_hash(a, b):
         return mix(a ^ mix(b))
 
 
-def ensemble_leg(la, steps, warmup=10):
+def ensemble_leg(la, steps, warmup=10, pinned=None):
     """Batched ensemble: 256 scenarios of log-uniform 1,024-65,536 objects (~4M in total), each
     a seeded population of the 16,384-object distributions; full force model, split
     precision, dt = 10 s, detection every step (10 km cutoff, 1 km threshold), burn-up below
@@ -308,7 +310,11 @@ def ensemble_leg(la, steps, warmup=10):
     with la.Engine(cfg) as eng:
         eng.set_objects(pop)
         eng.set_scenarios(off)
-        tuning = eng.autotune(repetitions=2)
+        if pinned:
+            eng.set_tuning(pinned)
+            tuning = dict(pinned)
+        else:
+            tuning = eng.autotune(repetitions=2)
         eng.simulate(warmup, tpcd=1, min_altitude_km=MIN_ALT)
         for graph in (1, 0):  # the same tuned context, steps replayed from a graph or launched eagerly
             eng.configure(step_graph=graph)
@@ -544,7 +550,9 @@ def main():
         out["adjoint"], grads = adjoint_leg(la)
         print(f"# adjoint: {json.dumps(out['adjoint'])}", file=sys.stderr, flush=True)
     if not a.no_ensemble:
-        out["ensemble"] = ensemble_leg(la, a.ensemble_steps)
+        etext = (a.ensemble_tuning or "").strip()
+        epinned = (json.loads(etext if etext.startswith("{") else Path(etext).read_text()) if etext else None)
+        out["ensemble"] = ensemble_leg(la, a.ensemble_steps, pinned=epinned)
         e = out["ensemble"]
         print(f"# ensemble: {e['object_steps_per_s']:.4g} object-steps/s (eager {e['eager_object_steps_per_s']:.4g}),"
               f" graph==eager {e['graph_equals_eager']}, collisions {e['collisions']}", file=sys.stderr, flush=True)
