@@ -44,6 +44,38 @@ def test_struct_layouts():
     assert la.PAIR_DTYPE.itemsize == 56
 
 
+# the ctypes mirrors of the header's structures
+MIRRORS = {"ladds_config": _abi.Config, "ladds_objects": _abi.Objects, "ladds_pair": _abi.Pair,
+           "ladds_logged_pair": _abi.LoggedPair, "ladds_run_stats": _abi.RunStats, "ladds_tuning": _abi.Tuning,
+           "ladds_adjoint_config": _abi.AdjointConfig, "ladds_device_objects": _abi.DeviceObjects}
+
+
+def test_ctypes_mirrors_match_the_header(tmp_path):
+    """Every ctypes mirror has the C structure's size and every field its offset, as gcc lays
+    the header out (fields appended to the ABI structures must be appended to both)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "ladds_amd.h"', "int main(void) {"]
+    for cname, py in MIRRORS.items():
+        lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-std=c99", "-I", str(HEADER.parent), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                        text=True).stdout.splitlines())
+    for cname, py in MIRRORS.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, f"{cname}.{fname}"
+
+
 def test_create_without_gpu_fails_loudly():
     if la.device_count() > 0:
         pytest.skip("a GPU is present")
