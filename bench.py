@@ -537,10 +537,11 @@ def main():
         return out
 
     kern = m["kernels"]
-    dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
-    r = roof(main_form, dom, kern[dom])
-    if r:
-        out["roofline"] = r
+    if kern:  # --profile-steps 0 times no kernel
+        dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
+        r = roof(main_form, dom, kern[dom])
+        if r:
+            out["roofline"] = r
     out["roofline_all"] = {form: {name: roof(form, name, k) for name, k in res[form]["kernels"].items()}
                            for form in res}
     out["tuning"] = {form: res[form]["tuning"] for form in res}

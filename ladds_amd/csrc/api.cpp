@@ -185,8 +185,8 @@ struct ladds_ctx {
 
   uint64_t step_count = 0;  // steps since t0
   // Environment segment staged by ladds_propagate: steps [env_first, env_first + env_len)
-  // of the clock (env_t0, env_dt) are in env64/env32 from index 0 (reused by later calls;
-  // ladds_simulate stages its own and clears env_len).
+  // of the clock (env_t0, env_dt) are in env64/env32 from index 0 (reused by later calls of
+  // ladds_propagate and ladds_simulate while they cover the steps; see env_window).
   double env_t0 = 0.0, env_dt = 0.0;
   uint64_t env_first = 0, env_len = 0;
   // ladds_propagate returned with its steps still running on `stream`; entry points that
@@ -581,6 +581,27 @@ int stage_env(ladds_ctx *ctx, int first, int count) {
   const double tau[3] = {ctx->c[0], ctx->c[0] + ctx->c[1], ctx->c[0] + ctx->c[1] + ctx->c[2]};
   HIPCHK(ctx, launch_substep_env(ctx->cfg.t0, ctx->cfg.dt, ctx->step_count, tau, first, count, ctx->env64,
                                  ctx->env32, ctx->stream));
+  return LADDS_OK;
+}
+
+// Offset in env64/env32 of the context's current step.  The segment staged last serves while
+// it covers the next `count` steps of the same clock (environments depend only on t0, dt and
+// the step index); otherwise a segment of kEnvSegment steps is staged from the current step,
+// on the context's stream, after the propagations that read the previous one.
+int env_window(ladds_ctx *ctx, int count, size_t *off) {
+  const bool cached = ctx->env_len && ctx->env_t0 == ctx->cfg.t0 && ctx->env_dt == ctx->cfg.dt &&
+                      ctx->step_count >= ctx->env_first &&
+                      ctx->step_count + (uint64_t)count <= ctx->env_first + ctx->env_len;
+  if (!cached) {
+    ctx->env_len = 0;
+    const int rc = stage_env(ctx, 0, kEnvSegment);
+    if (rc != LADDS_OK) return rc;
+    ctx->env_t0 = ctx->cfg.t0;
+    ctx->env_dt = ctx->cfg.dt;
+    ctx->env_first = ctx->step_count;
+    ctx->env_len = (uint64_t)kEnvSegment;
+  }
+  *off = (size_t)(ctx->step_count - ctx->env_first);
   return LADDS_OK;
 }
 
@@ -1356,18 +1377,8 @@ int ladds_propagate(ladds_ctx *ctx, int nsteps) {
     const int seg = std::min(kEnvSegment, nsteps - seg0);
     // The environment of a whole segment is staged once and serves the calls that follow
     // (integrate(false) once per iteration) until the clock leaves it.
-    const bool cached = ctx->env_len && ctx->env_t0 == ctx->cfg.t0 && ctx->env_dt == ctx->cfg.dt &&
-                        ctx->step_count >= ctx->env_first &&
-                        ctx->step_count + (uint64_t)seg <= ctx->env_first + ctx->env_len;
-    int rc = LADDS_OK;
-    if (!cached) {
-      rc = stage_env(ctx, 0, kEnvSegment);
-      ctx->env_t0 = ctx->cfg.t0;
-      ctx->env_dt = ctx->cfg.dt;
-      ctx->env_first = ctx->step_count;
-      ctx->env_len = rc == LADDS_OK ? (uint64_t)kEnvSegment : 0;
-    }
-    const size_t off = (size_t)(ctx->step_count - ctx->env_first);
+    size_t off = 0;
+    int rc = env_window(ctx, seg, &off);
     for (int s = 0; s < seg && rc == LADDS_OK; s += kChunk) {
       const int k = std::min(kChunk, seg - s);
       rc = launch_propagate_steps(ctx, off + (size_t)s, k, tiles && seg0 + s + k == nsteps, 0.0);
@@ -1470,7 +1481,6 @@ int snapshot_copy(ladds_ctx *ctx, SimSnapshot &s, bool save) {
     ctx->quiet = s.quiet;
     ctx->protected_objects = s.protected_objects;
     ctx->keys_valid = false;
-    ctx->env_len = 0;
   }
   return LADDS_OK;
 }
@@ -1961,11 +1971,11 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   }
   for (int seg0 = 0; seg0 < nsteps && ctx->n > 0; seg0 += kEnvSegment) {
     const int seg = std::min(kEnvSegment, nsteps - seg0);
-    // environments of the whole segment, in stream order after the previous segment's steps
-    // (they replace the segment ladds_propagate staged)
+    // environments of the whole segment: the staged segment when it covers these steps (an
+    // earlier call's), else staged here in stream order after the previous segment's steps
+    size_t env_off = 0;
     {
-      ctx->env_len = 0;
-      const int rc = stage_env(ctx, 0, seg);
+      const int rc = env_window(ctx, seg, &env_off);
       if (rc != LADDS_OK) {
         drop_graphs();
         return rc;
@@ -1977,8 +1987,10 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
       // epilogue: the pair scan of the segment's last step.
       const uint64_t base = first_iteration + (uint64_t)seg0;
       int rc = LADDS_OK;
-      hipError_t e = launch_set_step(ctx->d_step, 0u, base, ctx->stream);
-      if (e == hipSuccess) rc = launch_prop_part(ctx, cur, 0, 1, true, burnup_r2, nullptr, 0);
+      // the device step indexes the environments (from env_off) and labels the iteration
+      // (iter_base + step, modulo 2^64)
+      hipError_t e = launch_set_step(ctx->d_step, (uint32_t)env_off, base - (uint64_t)env_off, ctx->stream);
+      if (e == hipSuccess) rc = launch_prop_part(ctx, cur, env_off, 1, true, burnup_r2, nullptr, 0);
       cur = 1 - cur;
       for (int j = 0; j + 1 < seg && rc == LADDS_OK && e == hipSuccess; ++j) {
         e = hipGraphLaunch(graphs[cur], ctx->stream);
@@ -2011,14 +2023,14 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
       int rc;
       if (use_overlap) {
         const int nxt = (cur + 1) % 3;
-        rc = overlapped_iteration(ctx, cur, pending[nxt], (size_t)s, k, detect, burnup_r2,
+        rc = overlapped_iteration(ctx, cur, pending[nxt], env_off + (size_t)s, k, detect, burnup_r2,
                                   ctx->cfg.dt * (double)tpcd, digest, it, span);
         if (rc == LADDS_OK) {
           pending[nxt] = detect;
           cur = nxt;
         }
       } else {
-        rc = launch_propagate_steps(ctx, (size_t)s, k, detect, burnup_r2);
+        rc = launch_propagate_steps(ctx, env_off + (size_t)s, k, detect, burnup_r2);
         if (rc == LADDS_OK && detect)
           rc = launch_detection(ctx, true, ctx->cfg.dt * (double)tpcd, digest, it, nullptr, span);
       }

@@ -109,9 +109,9 @@ struct alignas(16) RecordF32 {
 // Device-side step position for graph replay: a captured step reads which environment
 // and iteration it is at from here, and its last kernel advances `step`.
 struct StepState {
-  uint32_t step;      // step within the current environment segment
+  uint32_t step;      // index of the step in the staged environment segment
   uint32_t pad;
-  uint64_t iter_base; // iteration of step 0 of the segment
+  uint64_t iter_base; // iteration label = iter_base + step (modulo 2^64)
 };
 
 // Device view of the object arrays.
