@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -50,6 +51,15 @@ struct HostReadback {
   unsigned long long log_count;
   SpawnState spawn;
 };
+// The scan counters and the pair-log count share one device block laid out as the start of
+// HostReadback, so one copy reads both back.
+struct DeviceReadback {
+  ScanCounters sc;
+  unsigned long long log_count;
+};
+static_assert(offsetof(HostReadback, log_count) == offsetof(DeviceReadback, log_count) &&
+                  sizeof(DeviceReadback) == offsetof(HostReadback, spawn),
+              "one copy reads the counters and the log count");
 
 // Pair-scan pipelines (ladds_config.scan_variant).
 enum : int { kVariantAuto = 0, kVariantAtomicSort = 1, kVariantPlaneTiles = 2 };
@@ -120,7 +130,8 @@ struct ladds_ctx {
   ScanCounters *counters = nullptr;
   // Pinned host copies of the counters, the log count and the spawn state: one
   // asynchronous read-back and one synchronisation at the end of ladds_simulate.
-  HostReadback *h_rb = nullptr;
+  HostReadback *h_rb = nullptr;         // pinned, coherent: kernels store into it (d_rb_host)
+  HostReadback *d_rb_host = nullptr;     // device address of h_rb
   // Pinned staging for pair-log entries returned by ladds_conjunctions, and for the
   // object states gathered by ladds_gather_state.
   LoggedPair *h_log = nullptr;
@@ -577,24 +588,24 @@ int alloc_env(ladds_ctx *ctx) {
 // Environments of steps [first, first + count) of the segment starting at the context's
 // current step, evaluated on the context's stream (ordered before the propagation that
 // reads them and after the one that read the previous segment's).
-int stage_env(ladds_ctx *ctx, int first, int count) {
+int stage_env(ladds_ctx *ctx, int first, int count, hipStream_t stream = nullptr) {
   const double tau[3] = {ctx->c[0], ctx->c[0] + ctx->c[1], ctx->c[0] + ctx->c[1] + ctx->c[2]};
   HIPCHK(ctx, launch_substep_env(ctx->cfg.t0, ctx->cfg.dt, ctx->step_count, tau, first, count, ctx->env64,
-                                 ctx->env32, ctx->stream));
+                                 ctx->env32, stream ? stream : ctx->stream));
   return LADDS_OK;
 }
 
 // Offset in env64/env32 of the context's current step.  The segment staged last serves while
 // it covers the next `count` steps of the same clock (environments depend only on t0, dt and
 // the step index); otherwise a segment of kEnvSegment steps is staged from the current step,
-// on the context's stream, after the propagations that read the previous one.
-int env_window(ladds_ctx *ctx, int count, size_t *off) {
+// on `stream` (default: the context's), after the propagations that read the previous one.
+int env_window(ladds_ctx *ctx, int count, size_t *off, hipStream_t stream = nullptr) {
   const bool cached = ctx->env_len && ctx->env_t0 == ctx->cfg.t0 && ctx->env_dt == ctx->cfg.dt &&
                       ctx->step_count >= ctx->env_first &&
                       ctx->step_count + (uint64_t)count <= ctx->env_first + ctx->env_len;
   if (!cached) {
     ctx->env_len = 0;
-    const int rc = stage_env(ctx, 0, kEnvSegment);
+    const int rc = stage_env(ctx, 0, kEnvSegment, stream);
     if (rc != LADDS_OK) return rc;
     ctx->env_t0 = ctx->cfg.t0;
     ctx->env_dt = ctx->cfg.dt;
@@ -704,6 +715,13 @@ int launch_detection(ladds_ctx *ctx, bool binned, double dt_interp, bool digest,
 }
 
 // Pipeline buffers 1 .. nbuf-1 of the overlapped loop, sized like buffer 0.
+// Stream `to` follows what is queued on `from` so far (event `ev`, recorded there).
+int follow(ladds_ctx *ctx, hipEvent_t ev, hipStream_t from, hipStream_t to) {
+  HIPCHK(ctx, hipEventRecord(ev, from));
+  HIPCHK(ctx, hipStreamWaitEvent(to, ev, 0));
+  return LADDS_OK;
+}
+
 int ensure_overlap_buffers(ladds_ctx *ctx, int nbuf) {
   for (int b = 1; b < nbuf; ++b) {
     PipeBuffer &pb = ctx->pipe[b];
@@ -773,15 +791,16 @@ uint32_t *arena_top_of(ladds_ctx *ctx, int b) { return b ? ctx->pipe[b].arena_to
 // Propagate from state buffer `cur` into `nxt` (on `stream`), writing the cell keys of the
 // result when `keys`.  With `step_state` the environments start at its step + step_offset.
 int launch_prop_into(ladds_ctx *ctx, int cur, int nxt, size_t env_offset, int nsteps, bool keys, double burnup_r2,
-                     const StepState *step_state, uint32_t step_offset) {
+                     const StepState *step_state, uint32_t step_offset, hipStream_t stream = nullptr) {
   const DeviceObjects in = state_view(ctx, cur), out = state_view(ctx, nxt);
   double cdt[4], ddt[3];
   for (int k = 0; k < 4; ++k) cdt[k] = ctx->c[k] * ctx->cfg.dt;
   for (int k = 0; k < 3; ++k) ddt[k] = ctx->dcoef[k] * ctx->cfg.dt;
-  const hipError_t e = timed(ctx, "propagate", [&] {
+  if (!stream) stream = ctx->stream;
+  const hipError_t e = timed_on(ctx, "propagate", stream, [&] {
     return launch_propagate(ctx->form, in, ctx->env64 + 3 * env_offset, ctx->env32 + 3 * env_offset, nsteps,
                             ctx->cfg.components, cdt, ddt, &ctx->grid, keys ? keys_of(ctx, nxt) : nullptr, nullptr,
-                            ctx->slot, burnup_r2, step_state, ctx->stream, &out, step_offset,
+                            ctx->slot, burnup_r2, step_state, stream, &out, step_offset,
                             &ctx->counters->burnups);
   });
   if (e != hipSuccess) return ctx->hipfail(e, "propagation launch");
@@ -841,22 +860,42 @@ constexpr size_t kPipelineStageBytes = 32 * 1024;
 // the propagation waits for that pair scan, which itself followed its partition.  So step
 // k+2's propagation, step k+1's partition and step k's pair scan can all run at once, and
 // the partition (bound by HBM traffic) no longer holds up the next propagation.
+//
+// `fill` (the call's first iteration, when it detects): nothing runs yet, so the
+// propagation, partition and pair scan all go on `stream2` in order, with no hop between
+// streams on the way to the first pair scan (each hop costs ~20 us of signalling); the next
+// propagation, on `stream`, waits for this one only.
 int overlapped_iteration(ladds_ctx *ctx, int cur, bool pending_next, size_t env_offset, int nsteps, bool detect,
-                         double burnup_r2, double dt_interp, bool digest, uint64_t iteration, uint64_t spawn_span) {
+                         double burnup_r2, double dt_interp, bool digest, uint64_t iteration, uint64_t spawn_span,
+                         bool fill = false) {
   const int nxt = (cur + 1) % 3;
-  if (pending_next) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_pair[nxt], 0));
+  const size_t stage = ctx->form == kFormFp64 ? 0 : kPipelineStageBytes;
+  if (fill && detect) {
+    int rc = launch_prop_into(ctx, cur, nxt, env_offset, nsteps, true, burnup_r2, nullptr, 0, ctx->stream2);
+    if (rc == LADDS_OK) rc = follow(ctx, ctx->ev_prop[nxt], ctx->stream2, ctx->stream);
+    if (rc == LADDS_OK) rc = launch_part_of(ctx, nxt, ctx->stream2, stage);
+    if (rc == LADDS_OK) rc = launch_pairs_of(ctx, nxt, dt_interp, digest, iteration, nullptr, ctx->stream2, spawn_span);
+    if (rc == LADDS_OK) HIPCHK(ctx, hipEventRecord(ctx->ev_pair[nxt], ctx->stream2));
+    return rc;
+  }
+  // A hipStreamWaitEvent on a pending event costs 10-23 us of host time (2-3 us for a
+  // launch), enough with three a step to make the host nearly as slow as the GPU: the wait
+  // for the pair scan that last read buffer nxt is left out once that scan is done.
+  if (pending_next) {
+    const hipError_t q = hipEventQuery(ctx->ev_pair[nxt]);  // (a pending event's answer is no error)
+    if (q == hipErrorNotReady)
+      HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_pair[nxt], 0));
+    else
+      HIPCHK(ctx, q);
+  }
   int rc = launch_prop_into(ctx, cur, nxt, env_offset, nsteps, detect, burnup_r2, nullptr, 0);
   if (rc != LADDS_OK || !detect) return rc;
-  HIPCHK(ctx, hipEventRecord(ctx->ev_prop[nxt], ctx->stream));
-  HIPCHK(ctx, hipStreamWaitEvent(ctx->stream3, ctx->ev_prop[nxt], 0));
-  rc = launch_part_of(ctx, nxt, ctx->stream3, ctx->form == kFormFp64 ? 0 : kPipelineStageBytes);
-  if (rc != LADDS_OK) return rc;
-  HIPCHK(ctx, hipEventRecord(ctx->ev_part[nxt], ctx->stream3));
-  HIPCHK(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_part[nxt], 0));
-  rc = launch_pairs_of(ctx, nxt, dt_interp, digest, iteration, nullptr, ctx->stream2, spawn_span);
-  if (rc != LADDS_OK) return rc;
-  HIPCHK(ctx, hipEventRecord(ctx->ev_pair[nxt], ctx->stream2));
-  return LADDS_OK;
+  rc = follow(ctx, ctx->ev_prop[nxt], ctx->stream, ctx->stream3);
+  if (rc == LADDS_OK) rc = launch_part_of(ctx, nxt, ctx->stream3, stage);
+  if (rc == LADDS_OK) rc = follow(ctx, ctx->ev_part[nxt], ctx->stream3, ctx->stream2);
+  if (rc == LADDS_OK) rc = launch_pairs_of(ctx, nxt, dt_interp, digest, iteration, nullptr, ctx->stream2, spawn_span);
+  if (rc == LADDS_OK) HIPCHK(ctx, hipEventRecord(ctx->ev_pair[nxt], ctx->stream2));
+  return rc;
 }
 
 // Software-pipelined step graph for state buffer b: the pair scan of the state in b (on a
@@ -994,16 +1033,24 @@ int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_pair[b], hipEventDisableTiming);
   }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_rb), sizeof(HostReadback), 0);
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void **>(&ctx->h_rb), sizeof(HostReadback),
+                      hipHostMallocMapped | hipHostMallocCoherent);
+  if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void **>(&ctx->d_rb_host), ctx->h_rb, 0);
   if (e == hipSuccess) e = upload_atmosphere();
   if (e == hipSuccess) e = configure_tile_kernels();
   if (e == hipSuccess) e = dalloc(&ctx->arena_top, kScanCounterWords);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
-  if (e == hipSuccess) e = dalloc(&ctx->log_count, 1);
-  if (e == hipSuccess) e = dalloc(&ctx->counters, 1);
+  if (e == hipSuccess) {
+    DeviceReadback *rb = nullptr;
+    e = dalloc(&rb, 1);
+    if (e == hipSuccess) {
+      ctx->counters = &rb->sc;  // the block's start: freed through this pointer
+      ctx->log_count = &rb->log_count;
+      e = hipMemset(rb, 0, sizeof(DeviceReadback));
+    }
+  }
   if (e == hipSuccess) e = dalloc(&ctx->n_alive, 1);
-  if (e == hipSuccess) e = hipMemset(ctx->log_count, 0, sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipMemset(ctx->counters, 0, sizeof(ScanCounters));
   if (e != hipSuccess) {
     ladds_destroy(ctx);
     return (e == hipErrorOutOfMemory) ? LADDS_E_NOMEM : LADDS_E_HIP;
@@ -1061,8 +1108,7 @@ int ladds_destroy(ladds_ctx *ctx) {
   hipFree(ctx->env64);
   hipFree(ctx->env32);
   hipFree(ctx->log);
-  hipFree(ctx->log_count);
-  hipFree(ctx->counters);
+  hipFree(ctx->counters);  // with the log count (one DeviceReadback block)
   hipFree(ctx->xbuf);
   hipFree(ctx->xcount);
   for (auto &kv : ctx->prof)
@@ -1797,8 +1843,7 @@ int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_
   // come back together; more entries than staged, or than the log holds, cost a second.
   uint64_t entries = 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
-    HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
-    HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(DeviceReadback), ctx->stream));  // with the log count
     int rc = launch_detection(ctx, ctx->keys_valid, dt_interp, false, 0);
     if (rc != LADDS_OK) return rc;
     if (!ctx->h_log) {
@@ -1808,9 +1853,8 @@ int ladds_conjunctions(ladds_ctx *ctx, double dt_interp, ladds_pair *hits, size_
     // the first 256 entries come back with the counters (a call that finds more copies the
     // rest after the first synchronisation)
     const size_t staged = std::min<size_t>({ctx->h_log_cap, ctx->log_cap, 256});
-    HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->sc, ctx->counters, sizeof(ScanCounters), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->log_count, ctx->log_count, sizeof(unsigned long long),
-                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->sc, ctx->counters, sizeof(DeviceReadback), hipMemcpyDeviceToHost,
+                               ctx->stream));  // counters and log count
     HIPCHK(ctx, hipMemcpyAsync(ctx->h_log, ctx->log, sizeof(LoggedPair) * staged, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipMemsetAsync(ctx->log_count, 0, sizeof(unsigned long long), ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1922,7 +1966,6 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   LADDS_SETTLE(ctx);
   if (stats) std::memset(stats, 0, sizeof(*stats));
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), ctx->stream));
   const bool digest = ctx->cfg.pair_output == LADDS_PAIRS_DIGEST;
   const double burnup_r = min_altitude_km > 0.0 ? min_altitude_km + kREarth : 0.0;
   const double burnup_r2 = burnup_r * burnup_r;  // Simulation.cpp:576,583
@@ -1937,6 +1980,21 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   const bool use_graph = ctx->cfg.step_graph != 0 && tpcd == 1 && ctx->variant == kVariantPlaneTiles &&
                          !ctx->profiling && ctx->n > 0 && nsteps > 0;
   const bool use_overlap = !use_graph && ctx->variant == kVariantPlaneTiles && ctx->n > 0 && nsteps > 0;
+  // The eager loop's first iteration runs on stream2 (overlapped_iteration's `fill`), and so
+  // does what precedes it here (counter reset, spawn state, environments), after anything
+  // still queued on the context's stream.
+  hipStream_t head = ctx->stream;
+  if (use_overlap) {
+    head = ctx->stream2;
+    const hipError_t q = hipStreamQuery(ctx->stream);  // (a busy stream's answer is no error)
+    if (q == hipErrorNotReady) {
+      const int rc = follow(ctx, ctx->ev_join, ctx->stream, head);
+      if (rc != LADDS_OK) return rc;
+    } else {
+      HIPCHK(ctx, q);
+    }
+  }
+  HIPCHK(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(ScanCounters), head));
   // Spawn protection on the device while objects with a parent id exist: every detection
   // accounts the iterations since the previous one (span), trailing iterations at the end.
   const bool spawn = ctx->protected_objects && ctx->n > 0 && nsteps > 0;
@@ -1944,11 +2002,11 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   if (spawn) {
     if (!ctx->d_spawn) HIPCHK(ctx, dalloc(&ctx->d_spawn, 1));
     ctx->h_rb->spawn = SpawnState{ctx->quiet, 0ull, 0ull, 0ull};
-    HIPCHK(ctx, hipMemcpyAsync(ctx->d_spawn, &ctx->h_rb->spawn, sizeof(SpawnState), hipMemcpyHostToDevice,
-                               ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_spawn, &ctx->h_rb->spawn, sizeof(SpawnState), hipMemcpyHostToDevice, head));
   }
   int cur = 0;                              // state buffer holding the latest state
   bool pending[3] = {false, false, false};  // a pair scan reading buffer b may still run on stream2
+  bool fill = use_overlap;                  // the next overlapped iteration is the call's first
   if (use_overlap || use_graph) {
     const int rc = ensure_overlap_buffers(ctx, use_graph ? 2 : 3);
     if (rc != LADDS_OK) return rc;
@@ -1975,7 +2033,7 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     // earlier call's), else staged here in stream order after the previous segment's steps
     size_t env_off = 0;
     {
-      const int rc = env_window(ctx, seg, &env_off);
+      const int rc = env_window(ctx, seg, &env_off, seg0 == 0 ? head : ctx->stream);
       if (rc != LADDS_OK) {
         drop_graphs();
         return rc;
@@ -2023,8 +2081,12 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
       int rc;
       if (use_overlap) {
         const int nxt = (cur + 1) % 3;
-        rc = overlapped_iteration(ctx, cur, pending[nxt], env_off + (size_t)s, k, detect, burnup_r2,
-                                  ctx->cfg.dt * (double)tpcd, digest, it, span);
+        // the prologue ran on stream2: order this propagation after it
+        rc = (fill && !detect) ? follow(ctx, ctx->ev_join, ctx->stream2, ctx->stream) : LADDS_OK;
+        if (rc == LADDS_OK)
+          rc = overlapped_iteration(ctx, cur, pending[nxt], env_off + (size_t)s, k, detect, burnup_r2,
+                                    ctx->cfg.dt * (double)tpcd, digest, it, span, fill);
+        fill = false;
         if (rc == LADDS_OK) {
           pending[nxt] = detect;
           cur = nxt;
@@ -2040,23 +2102,29 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     ctx->step_count += (uint64_t)seg;
   }
   // join the pair-scan stream, account trailing iterations, read the counters back: one
-  // synchronisation
-  if (use_overlap || use_graph) {
-    HIPCHK(ctx, hipEventRecord(ctx->ev_join, ctx->stream2));
-    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-  }
+  // synchronisation.  When the eager loop's last iteration detected, its pair scan (stream2)
+  // follows every launch of the call (its partition followed its propagation, which followed
+  // the others on the context's stream): the readback goes on stream2, with no join.
   const uint64_t end_it = first_iteration + (uint64_t)nsteps;
-  if (spawn && end_it > next_acc) {
+  const bool trailing = spawn && end_it > next_acc;
+  hipStream_t tail = ctx->stream;
+  if (use_overlap && pending[cur] && !trailing) {
+    tail = ctx->stream2;
+  } else if (use_overlap || use_graph) {
+    const int rc = follow(ctx, ctx->ev_join, ctx->stream2, ctx->stream);
+    if (rc != LADDS_OK) return rc;
+  }
+  if (trailing) {
     HIPCHK(ctx, spawn_pre(ctx, end_it - next_acc, ctx->stream));
     HIPCHK(ctx, spawn_post(ctx, end_it - next_acc, false, ctx->stream));
   }
-  HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->sc, ctx->counters, sizeof(ScanCounters), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->log_count, ctx->log_count, sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost, ctx->stream));
+  static_assert(sizeof(DeviceReadback) % 8 == 0, "whole words");
+  HIPCHK(ctx, launch_publish(reinterpret_cast<const unsigned long long *>(ctx->counters),
+                             reinterpret_cast<unsigned long long *>(&ctx->d_rb_host->sc),
+                             (int)(sizeof(DeviceReadback) / 8), tail));
   if (spawn)
-    HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->spawn, ctx->d_spawn, sizeof(SpawnState), hipMemcpyDeviceToHost,
-                               ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&ctx->h_rb->spawn, ctx->d_spawn, sizeof(SpawnState), hipMemcpyDeviceToHost, tail));
+  HIPCHK(ctx, hipStreamSynchronize(tail));
   drop_graphs();
   if ((use_overlap || use_graph) && cur != 0) {  // the final state is in another buffer: make it the context's own
     PipeBuffer &pb = ctx->pipe[cur];

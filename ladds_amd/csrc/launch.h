@@ -18,6 +18,8 @@ hipError_t launch_propagate(int form, const DeviceObjects &o, const SubstepEnv<d
 // Graph replay bookkeeping: set the step position, advance it by one step.
 hipError_t launch_set_step(StepState *st, uint32_t step, uint64_t iter_base, hipStream_t stream);
 hipError_t launch_advance_step(StepState *st, hipStream_t stream);
+// Copies `words` 8-byte words from device memory to mapped host memory (one workgroup).
+hipError_t launch_publish(const unsigned long long *src, unsigned long long *dst, int words, hipStream_t stream);
 
 hipError_t launch_bin_count(int form, const DeviceObjects &o, const GridParams &g, uint32_t *count, uint32_t *slot,
                             hipStream_t s);
