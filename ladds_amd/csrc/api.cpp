@@ -854,20 +854,21 @@ int launch_pairs_of(ladds_ctx *ctx, int b, double dt_interp, bool digest, uint64
 constexpr size_t kPipelineStageBytes = 32 * 1024;
 
 // One iteration of the eager overlapped loop: propagate buffer `cur` into nxt = cur + 1
-// (mod 3) on `stream`; when `detect`, partition nxt on `stream3` once the propagation is
-// done, and scan its pairs on `stream2` once the partition is done.  Buffer nxt was last
-// read by the partition and the pair scan of the step three iterations back (`pending_next`):
-// the propagation waits for that pair scan, which itself followed its partition.  So step
-// k+2's propagation, step k+1's partition and step k's pair scan can all run at once, and
-// the partition (bound by HBM traffic) no longer holds up the next propagation.
+// (mod 3) on `stream` and, when `detect`, partition nxt on `stream3` once the propagation is
+// done; its pair scan (issue_scan, on `stream2`) is queued by the next iteration.  Buffer
+// nxt was last read by the partition and the pair scan of the step three iterations back
+// (`pending_next`): the propagation waits for that pair scan, which itself followed its
+// partition.  So step k+2's propagation, step k+1's partition and step k's pair scan can
+// all run at once, and the partition (bound by HBM traffic) no longer holds up the next
+// propagation.
 //
 // `fill` (the call's first iteration, when it detects): nothing runs yet, so the
 // propagation, partition and pair scan all go on `stream2` in order, with no hop between
-// streams on the way to the first pair scan (each hop costs ~20 us of signalling); the next
-// propagation, on `stream`, waits for this one only.
+// streams on the way to the first pair scan; the next propagation, on `stream`, waits for
+// this one only.
 int overlapped_iteration(ladds_ctx *ctx, int cur, bool pending_next, size_t env_offset, int nsteps, bool detect,
                          double burnup_r2, double dt_interp, bool digest, uint64_t iteration, uint64_t spawn_span,
-                         bool fill = false) {
+                         bool fill) {
   const int nxt = (cur + 1) % 3;
   const size_t stage = ctx->form == kFormFp64 ? 0 : kPipelineStageBytes;
   if (fill && detect) {
@@ -892,9 +893,33 @@ int overlapped_iteration(ladds_ctx *ctx, int cur, bool pending_next, size_t env_
   if (rc != LADDS_OK || !detect) return rc;
   rc = follow(ctx, ctx->ev_prop[nxt], ctx->stream, ctx->stream3);
   if (rc == LADDS_OK) rc = launch_part_of(ctx, nxt, ctx->stream3, stage);
-  if (rc == LADDS_OK) rc = follow(ctx, ctx->ev_part[nxt], ctx->stream3, ctx->stream2);
-  if (rc == LADDS_OK) rc = launch_pairs_of(ctx, nxt, dt_interp, digest, iteration, nullptr, ctx->stream2, spawn_span);
-  if (rc == LADDS_OK) HIPCHK(ctx, hipEventRecord(ctx->ev_pair[nxt], ctx->stream2));
+  if (rc == LADDS_OK) HIPCHK(ctx, hipEventRecord(ctx->ev_part[nxt], ctx->stream3));
+  return rc;
+}
+
+// The pair scan of a step whose partition overlapped_iteration queued, issued one
+// iteration later.
+struct PendingScan {
+  bool valid = false;
+  int b = 0;
+  double dt_interp = 0.0;
+  uint64_t iteration = 0, span = 0;
+};
+
+// Queue the pair scan of buffer d.b on `stream2` once its partition has finished, with no
+// wait packet.  A wait packet in front of the pair kernel (hipStreamWaitEvent on the
+// partition) cost ~12 us between consecutive pair scans at 1M objects even though the
+// partition had finished ~50 us before: the host waits instead (the partition of step k+1
+// ends while the pair scan of step k still runs, so the next pair scan is queued before
+// the pair stream runs dry).
+int issue_scan(ladds_ctx *ctx, const PendingScan &d, bool digest) {
+  for (;;) {
+    const hipError_t q = hipEventQuery(ctx->ev_part[d.b]);
+    if (q == hipSuccess) break;
+    if (q != hipErrorNotReady) return ctx->hipfail(q, "partition event");
+  }
+  int rc = launch_pairs_of(ctx, d.b, d.dt_interp, digest, d.iteration, nullptr, ctx->stream2, d.span);
+  if (rc == LADDS_OK) HIPCHK(ctx, hipEventRecord(ctx->ev_pair[d.b], ctx->stream2));
   return rc;
 }
 
@@ -2007,6 +2032,7 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   int cur = 0;                              // state buffer holding the latest state
   bool pending[3] = {false, false, false};  // a pair scan reading buffer b may still run on stream2
   bool fill = use_overlap;                  // the next overlapped iteration is the call's first
+  PendingScan scan;                         // pair scan still to queue (overlapped loop)
   if (use_overlap || use_graph) {
     const int rc = ensure_overlap_buffers(ctx, use_graph ? 2 : 3);
     if (rc != LADDS_OK) return rc;
@@ -2086,6 +2112,13 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
         if (rc == LADDS_OK)
           rc = overlapped_iteration(ctx, cur, pending[nxt], env_off + (size_t)s, k, detect, burnup_r2,
                                     ctx->cfg.dt * (double)tpcd, digest, it, span, fill);
+        // the previous iteration's pair scan, then this one's (the first iteration's ran)
+        if (rc == LADDS_OK && scan.valid) rc = issue_scan(ctx, scan, digest);
+        scan.valid = rc == LADDS_OK && detect && !fill;
+        scan.b = nxt;
+        scan.dt_interp = ctx->cfg.dt * (double)tpcd;
+        scan.iteration = it;
+        scan.span = span;
         fill = false;
         if (rc == LADDS_OK) {
           pending[nxt] = detect;
@@ -2100,6 +2133,10 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
       s += k;
     }
     ctx->step_count += (uint64_t)seg;
+  }
+  if (scan.valid) {  // the last detection's pair scan
+    const int rc = issue_scan(ctx, scan, digest);
+    if (rc != LADDS_OK) return rc;
   }
   // join the pair-scan stream, account trailing iterations, read the counters back: one
   // synchronisation.  When the eager loop's last iteration detected, its pair scan (stream2)
