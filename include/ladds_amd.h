@@ -301,12 +301,16 @@ typedef struct ladds_tuning {
   int32_t persistent;    /* pair kernel: 0 one workgroup per plane pair; 1 as many         */
                          /* workgroups as the CUs hold, taking plane pairs from a queue   */
                          /* per XCD until none is left                                     */
+  int32_t pair_streams;  /* ladds_simulate's pair scans: 1 stream (each starts after the  */
+                         /* previous ends) or 2 alternating (one starts while the other  */
+                         /* drains; 1 while spawn protection is active); 0: 2 for the    */
+                         /* split forms, 1 for FP64.  Tuned only with tune_simulate       */
 } ladds_tuning;
 
 LADDS_API int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuning *out);
 /* Apply a shape that ladds_autotune chose earlier (cell_size_km, plane_width_km,
  * plane_bits, pair_threads, tile_size, chunks, workgroups_per_cu, plane_order,
- * ensemble_plane_objects, row_cells, bucket_cells, persistent; the timing fields are
+ * ensemble_plane_objects, row_cells, bucket_cells, persistent, pair_streams; the timing fields are
  * ignored) without timing anything -- the counterpart of running LADDS with the
  * calibrated AutoPas configuration that tuning mode writes out (Simulation.cpp:404-407,
  * dumpCalibratedConfig).  A value outside the tuner's range is LADDS_E_INVALID. */

@@ -464,7 +464,8 @@ class Engine:
                 "pair_threads": t.pair_threads, "tile_size": t.tile_size, "chunks": t.chunks,
                 "workgroups_per_cu": t.workgroups_per_cu, "plane_order": t.plane_order,
                 "ensemble_plane_objects": t.ensemble_plane_objects, "row_cells": t.row_cells,
-                "bucket_cells": t.bucket_cells, "persistent": t.persistent, "scan_us": t.scan_us,
+                "bucket_cells": t.bucket_cells, "persistent": t.persistent, "pair_streams": t.pair_streams,
+                "scan_us": t.scan_us,
                 "default_us": t.default_us, "tried": t.tried}
 
     def set_tuning(self, tuning: dict):
@@ -478,6 +479,7 @@ class Engine:
         t.row_cells = int(tuning.get("row_cells", 0))
         t.bucket_cells = int(tuning.get("bucket_cells", 0))
         t.persistent = int(tuning.get("persistent", 0))
+        t.pair_streams = int(tuning.get("pair_streams", 0))
         self._check(lib.ladds_set_tuning(self._h, C.byref(t)), "set_tuning")
 
     # -- measurement ----------------------------------------------------

@@ -115,6 +115,7 @@ class Tuning(C.Structure):
         ("row_cells", C.c_int32),
         ("bucket_cells", C.c_int32),
         ("persistent", C.c_int32),
+        ("pair_streams", C.c_int32),
     ]
 
 

@@ -175,6 +175,7 @@ struct ladds_ctx {
   int tune_rows = 0;       // bucket row height along y in cells: 2 or 1 (0: 1)
   int tune_bcells = 0;     // bucket width along z in cells: 2 or 1 (0: 1)
   int tune_persistent = 0; // 1: pair kernel as persistent workgroups with per-XCD queues
+  int tune_pair_streams = 0; // pair scans of the overlapped loop on 1 or 2 streams (0: by form)
 
   StepState *d_step = nullptr;  // graph replay position (ladds_config.step_graph)
 
@@ -184,6 +185,8 @@ struct ladds_ctx {
   // the partition on `stream`.  Buffer 0 is the context's own arrays (`d`, `slot`, `rec`,
   // `tps`, `scratch`, `arena_top`); pipe[1] and pipe[2] hold the others.
   hipStream_t stream2 = nullptr, stream3 = nullptr;
+  hipStream_t stream4 = nullptr;   // second pair-scan stream (pair_streams 2)
+  hipEvent_t ev_join4 = nullptr;   // end of ladds_simulate: join of the two pair-scan streams
   hipEvent_t ev_prop[3] = {nullptr, nullptr, nullptr}, ev_part[3] = {nullptr, nullptr, nullptr},
              ev_pair[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev_join = nullptr;  // end of ladds_simulate: stream waits for stream2
@@ -906,21 +909,33 @@ struct PendingScan {
   uint64_t iteration = 0, span = 0;
 };
 
-// Queue the pair scan of buffer d.b on `stream2` once its partition has finished, with no
+// Queue the pair scan of buffer d.b on `ps` once its partition has finished, with no
 // wait packet.  A wait packet in front of the pair kernel (hipStreamWaitEvent on the
 // partition) cost ~12 us between consecutive pair scans at 1M objects even though the
 // partition had finished ~50 us before: the host waits instead (the partition of step k+1
 // ends while the pair scan of step k still runs, so the next pair scan is queued before
 // the pair stream runs dry).
-int issue_scan(ladds_ctx *ctx, const PendingScan &d, bool digest) {
+int issue_scan(ladds_ctx *ctx, const PendingScan &d, bool digest, hipStream_t ps) {
   for (;;) {
     const hipError_t q = hipEventQuery(ctx->ev_part[d.b]);
     if (q == hipSuccess) break;
     if (q != hipErrorNotReady) return ctx->hipfail(q, "partition event");
   }
-  int rc = launch_pairs_of(ctx, d.b, d.dt_interp, digest, d.iteration, nullptr, ctx->stream2, d.span);
-  if (rc == LADDS_OK) HIPCHK(ctx, hipEventRecord(ctx->ev_pair[d.b], ctx->stream2));
+  int rc = launch_pairs_of(ctx, d.b, d.dt_interp, digest, d.iteration, nullptr, ps, d.span);
+  if (rc == LADDS_OK) HIPCHK(ctx, hipEventRecord(ctx->ev_pair[d.b], ps));
   return rc;
+}
+
+// Pair-scan streams of the overlapped loop.  On one stream each pair scan starts only after
+// the previous one has ended, and ~12 us pass between them (its last workgroups drain, the
+// next launch ramps up).  Alternating two streams lets a pair scan start while the previous
+// one drains; the scans share only atomic counters and the pair log.  Measured at 1M
+// objects: split 72.5 -> 69.0 us per step; FP64 101 -> 104 us (two 80 KB FP64 pair
+// workgroups fill a CU's LDS, so the overlapping scans only contend).  Spawn protection
+// accounts the scans in order: one stream then.
+int pair_streams(const ladds_ctx *ctx) {
+  if (ctx->tune_pair_streams) return ctx->tune_pair_streams;
+  return ctx->form == kFormFp64 ? 1 : 2;
 }
 
 // Software-pipelined step graph for state buffer b: the pair scan of the state in b (on a
@@ -1052,6 +1067,8 @@ int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream4, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join4, hipEventDisableTiming);
   for (int b = 0; b < 3 && e == hipSuccess; ++b) {
     e = hipEventCreateWithFlags(&ctx->ev_prop[b], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_part[b], hipEventDisableTiming);
@@ -1095,6 +1112,7 @@ int ladds_destroy(ladds_ctx *ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
   if (ctx->stream3) hipStreamSynchronize(ctx->stream3);
+  if (ctx->stream4) hipStreamSynchronize(ctx->stream4);
   for (PipeBuffer &pb : ctx->pipe) {
     for (int k = 0; k < 3; ++k) {
       hipFree(pb.r[k]);
@@ -1113,12 +1131,14 @@ int ladds_destroy(ladds_ctx *ctx) {
     if (ctx->ev_pair[b]) hipEventDestroy(ctx->ev_pair[b]);
   }
   if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+  if (ctx->ev_join4) hipEventDestroy(ctx->ev_join4);
   if (ctx->h_rb) hipHostFree(ctx->h_rb);
   if (ctx->h_log) hipHostFree(ctx->h_log);
   if (ctx->h_x) hipHostFree(ctx->h_x);
   hipFree(ctx->d_spawn);
   if (ctx->stream2) hipStreamDestroy(ctx->stream2);
   if (ctx->stream3) hipStreamDestroy(ctx->stream3);
+  if (ctx->stream4) hipStreamDestroy(ctx->stream4);
   free_objects(ctx->d);
   free_objects(ctx->d_alt);
   hipFree(ctx->count);
@@ -1640,15 +1660,16 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   ctx->tune_rows = 0;
   ctx->tune_bcells = 0;
   ctx->tune_persistent = 0;
+  ctx->tune_pair_streams = 0;
   int rc = setup_grid(ctx, n, rx.data());
   if (rc != LADDS_OK) return rc;
   const double base_cell = 1.0 / ctx->grid.inv_cell;
   const bool ensemble = ctx->tiles.nscen > 1;
   // current point of the search: {plane bits (-1: sized from the objects per plane), threads,
   // tile, chunks, workgroups per CU, dispatch order, ensemble objects per plane (0: 64), row
-  // height and bucket width in cells, persistent pair kernel}
+  // height and bucket width in cells, persistent pair kernel, pair-scan streams (0: by form)}
   double cur_cell = base_cell, cur_fx = 1.0;
-  int cur[10] = {-1, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk, 0, 0, 0, 1, 1, 0};
+  int cur[11] = {-1, (int)ctx->tiles.pair_threads, (int)ctx->tiles.tile_size, (int)ctx->tiles.nchunk, 0, 0, 0, 1, 1, 0, 0};
   double default_us = 0.0;
   // the objective: one detection, or (ladds_config.tune_simulate) a burst of simulate steps
   auto measure = [&](double *t) {
@@ -1672,6 +1693,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     ctx->tune_rows = v[7];
     ctx->tune_bcells = v[8];
     ctx->tune_persistent = v[9];
+    ctx->tune_pair_streams = v[10];
     return setup_grid(ctx, n, rx.data());
   };
   // Coordinate descent, one dimension at a time, keeping every improvement: chunks per
@@ -1689,22 +1711,24 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   const int ens_opts[] = {32, 64, 128, 256, 512};
   struct Cand {
     double cell, fx;
-    std::array<int, 10> v;
+    std::array<int, 11> v;
   };
   // dimensions in search order (9: bucket shape, 10: persistent pair kernel), one pass (two
   // for ensembles), then 11: cell size and buckets per plane together -- one at a time, a
   // cell size that ties at the current bucket count is kept although it loses with more
   // buckets (measured on the 256-scenario ensemble: 12.6 km cells with 512 buckets per
-  // plane 6% faster than the 10.1 or 15.2 km cells with 128 that the passes stopped at)
+  // plane 6% faster than the 10.1 or 15.2 km cells with 128 that the passes stopped at);
+  // last, with the simulate objective only, 12: one or two pair-scan streams
   const int dims[] = {0, 1, 9, 2, 3, 4, 5, 6, 7, 10, 8};
   constexpr int kDims = (int)(sizeof(dims) / sizeof(dims[0]));
   std::vector<int> order(dims, dims + kDims);
   if (ensemble) order.insert(order.end(), dims, dims + kDims);
   order.push_back(11);
+  if (ctx->cfg.tune_simulate) order.push_back(12);  // (a single detection cannot see it)
   for (const int dim : order) {
     if (dim == 8 && !ensemble) continue;
     std::vector<Cand> cands;
-    std::array<int, 10> c{cur[0], cur[1], cur[2], cur[3], cur[4], cur[5], cur[6], cur[7], cur[8], cur[9]};
+    std::array<int, 11> c{cur[0], cur[1], cur[2], cur[3], cur[4], cur[5], cur[6], cur[7], cur[8], cur[9], cur[10]};
     if (dim == 0)
       for (int k : chunk_opts) { c[3] = k; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 1)
@@ -1712,7 +1736,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     // cell size and plane shape change the objects per bucket: each option is tried with the
     // current buckets per plane and with the count sized from it again (they interact: on the
     // 256-scenario ensemble a fixed count kept the search in a 6% slower basin)
-    std::array<int, 10> c_auto = c;
+    std::array<int, 11> c_auto = c;
     c_auto[0] = -1;
     if (dim == 2)
       for (double f : cell_opts) {
@@ -1742,6 +1766,8 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
       for (int e : ens_opts) { c[6] = e; c[0] = -1; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 10)
       for (int p = 0; p < 2; ++p) { c[9] = p; cands.push_back({cur_cell, cur_fx, c}); }
+    if (dim == 12)  // pair-scan streams of the overlapped loop
+      for (int p = 1; p <= 2; ++p) { c[10] = p; cands.push_back({cur_cell, cur_fx, c}); }
     if (dim == 11) {
       rc = apply(cur_cell, cur_fx, cur);
       if (rc != LADDS_OK) return rc;
@@ -1774,7 +1800,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
         best_us = us;
         cur_cell = cand.cell;
         cur_fx = cand.fx;
-        for (int k = 0; k < 10; ++k) cur[k] = cand.v[k];
+        for (int k = 0; k < 11; ++k) cur[k] = cand.v[k];
       }
     }
   }
@@ -1796,6 +1822,7 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
     out->row_cells = ctx->tiles.half_rows ? 2 : 1;
     out->bucket_cells = ctx->tiles.half_buckets ? 2 : 1;
     out->persistent = ctx->tiles.persistent_groups ? 1 : 0;
+    out->pair_streams = pair_streams(ctx);
     out->scan_us = best_us;
     out->default_us = default_us;
     out->tried = tried;
@@ -1834,7 +1861,8 @@ int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *t) {
                   t->chunks <= 32 && t->workgroups_per_cu >= 0 && t->workgroups_per_cu <= 8 &&
                   (t->plane_order == 0 || t->plane_order == 1) && t->ensemble_plane_objects >= 0 &&
                   t->ensemble_plane_objects <= 65536 && t->row_cells >= 0 && t->row_cells <= 2 &&
-                  t->bucket_cells >= 0 && t->bucket_cells <= 2 && (t->persistent == 0 || t->persistent == 1);
+                  t->bucket_cells >= 0 && t->bucket_cells <= 2 && (t->persistent == 0 || t->persistent == 1) &&
+                  t->pair_streams >= 0 && t->pair_streams <= 2;
   if (!ok) return ctx->fail(LADDS_E_INVALID, "tuning values outside the tuner's range");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   ctx->tune_cell_km = t->cell_size_km;
@@ -1849,6 +1877,7 @@ int ladds_set_tuning(ladds_ctx *ctx, const ladds_tuning *t) {
   ctx->tune_rows = t->row_cells;
   ctx->tune_bcells = t->bucket_cells;
   ctx->tune_persistent = t->persistent;
+  ctx->tune_pair_streams = t->pair_streams;
   if (ctx->n == 0) return LADDS_OK;  // applied at the next ladds_set_objects
   std::vector<double> rx(ctx->n);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -2033,6 +2062,8 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
   bool pending[3] = {false, false, false};  // a pair scan reading buffer b may still run on stream2
   bool fill = use_overlap;                  // the next overlapped iteration is the call's first
   PendingScan scan;                         // pair scan still to queue (overlapped loop)
+  const bool two_streams = !spawn && pair_streams(ctx) == 2;
+  bool alt = false, used4 = false, last4 = false;  // next scan on stream4; stream4 used; last scan there
   if (use_overlap || use_graph) {
     const int rc = ensure_overlap_buffers(ctx, use_graph ? 2 : 3);
     if (rc != LADDS_OK) return rc;
@@ -2113,7 +2144,12 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
           rc = overlapped_iteration(ctx, cur, pending[nxt], env_off + (size_t)s, k, detect, burnup_r2,
                                     ctx->cfg.dt * (double)tpcd, digest, it, span, fill);
         // the previous iteration's pair scan, then this one's (the first iteration's ran)
-        if (rc == LADDS_OK && scan.valid) rc = issue_scan(ctx, scan, digest);
+        if (rc == LADDS_OK && scan.valid) {
+          rc = issue_scan(ctx, scan, digest, alt ? ctx->stream4 : ctx->stream2);
+          used4 |= alt;
+          last4 = alt;
+          alt = two_streams && !alt;
+        }
         scan.valid = rc == LADDS_OK && detect && !fill;
         scan.b = nxt;
         scan.dt_interp = ctx->cfg.dt * (double)tpcd;
@@ -2135,20 +2171,28 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     ctx->step_count += (uint64_t)seg;
   }
   if (scan.valid) {  // the last detection's pair scan
-    const int rc = issue_scan(ctx, scan, digest);
+    const int rc = issue_scan(ctx, scan, digest, alt ? ctx->stream4 : ctx->stream2);
     if (rc != LADDS_OK) return rc;
+    used4 |= alt;
+    last4 = alt;
   }
-  // join the pair-scan stream, account trailing iterations, read the counters back: one
-  // synchronisation.  When the eager loop's last iteration detected, its pair scan (stream2)
-  // follows every launch of the call (its partition followed its propagation, which followed
-  // the others on the context's stream): the readback goes on stream2, with no join.
+  // join the pair-scan streams, account trailing iterations, read the counters back: one
+  // synchronisation.  When the eager loop's last iteration detected, its pair scan follows
+  // every launch of the call but the other pair stream's (its partition followed its
+  // propagation, which followed the others on the context's stream): the readback goes on
+  // that pair stream, after the other one.
   const uint64_t end_it = first_iteration + (uint64_t)nsteps;
   const bool trailing = spawn && end_it > next_acc;
   hipStream_t tail = ctx->stream;
   if (use_overlap && pending[cur] && !trailing) {
-    tail = ctx->stream2;
+    tail = last4 ? ctx->stream4 : ctx->stream2;
+    if (used4) {
+      const int rc = follow(ctx, ctx->ev_join4, last4 ? ctx->stream2 : ctx->stream4, tail);
+      if (rc != LADDS_OK) return rc;
+    }
   } else if (use_overlap || use_graph) {
-    const int rc = follow(ctx, ctx->ev_join, ctx->stream2, ctx->stream);
+    int rc = follow(ctx, ctx->ev_join, ctx->stream2, ctx->stream);
+    if (rc == LADDS_OK && used4) rc = follow(ctx, ctx->ev_join4, ctx->stream4, ctx->stream);
     if (rc != LADDS_OK) return rc;
   }
   if (trailing) {

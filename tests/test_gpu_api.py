@@ -152,6 +152,8 @@ def test_set_tuning_applies_a_saved_shape():
             eng.set_tuning(dict(t2, bucket_cells=-1))
         with pytest.raises(la.LaddsError):
             eng.set_tuning(dict(t2, persistent=2))
+        with pytest.raises(la.LaddsError):
+            eng.set_tuning(dict(t2, pair_streams=3))
     oh, _ = ol.pairs(ol.ObjectSet(pop.r(), pop.v(), pop.radius_m, pop.activity, pop.id, pop.parent),
                      ol.functor_cfg(10.0, 10.0, threshold_mode=1, threshold_km=1.0))
     key = lambda p: sorted(zip(p["id_a"].tolist(), p["id_b"].tolist()))

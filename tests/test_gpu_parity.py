@@ -350,6 +350,41 @@ def test_simulate_pipeline_equals_step_by_step(steps, tpcd, precision):
     assert np.array_equal(r1, r0) and np.array_equal(v1, v0)
 
 
+@pytest.mark.parametrize("streams", [1, 2])
+@pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
+def test_pair_streams_keep_the_pipeline_result(precision, streams):
+    """ladds_simulate's pair scans on one stream, or alternating over two so that one starts
+    while the previous drains (ladds_tuning.pair_streams): the pair log and the final state
+    equal, bit for bit, those of the same steps run one call at a time; detection every 1
+    and 2 iterations, step counts ending in each pair stream."""
+    pop = _cluster_population(29)
+    cfg = dict(dt=10.0, components=la.ALL_COMPONENTS, precision=precision, cutoff_km=10.0,
+               threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, log_capacity=1 << 22)
+    for steps, tpcd in ((24, 1), (25, 1), (27, 2)):
+        with engine(**cfg) as eng:
+            eng.set_objects(pop)
+            t = eng.autotune(repetitions=1)
+            eng.set_tuning(dict(t, pair_streams=streams))
+            eng.simulate(steps, first_iteration=0, tpcd=tpcd)
+            log = eng.fetch_log()
+            r1, v1 = eng.state()
+        ref = []
+        with engine(**cfg) as eng:
+            eng.set_objects(pop)
+            for it in range(steps):
+                eng.propagate(1)
+                if it % tpcd == 0:
+                    hits, ev = eng.conjunctions(dt_interp=10.0 * tpcd)
+                    for arr, evaded in ((hits, 0), (ev, 1)):
+                        ref += [(it, int(p["id_a"]), int(p["id_b"]), _d2_bits(p["d2"]), evaded) for p in arr]
+            r0, v0 = eng.state()
+        got = [(int(e["iteration"]), int(e["pair"]["id_a"]), int(e["pair"]["id_b"]), _d2_bits(e["pair"]["d2"]),
+                int(e["evaded"])) for e in log]
+        same = sorted(got) == sorted(ref)
+        assert len(got) > 1000 and same, (steps, tpcd, len(got), len(ref))
+        assert np.array_equal(r1, r0) and np.array_equal(v1, v0), (steps, tpcd)
+
+
 @pytest.mark.parametrize("graph", [0, 1])
 def test_split_calls_reuse_the_staged_environments(graph):
     """Sun, Moon and Earth-rotation environments depend only on the clock (t0, dt, step), so a
