@@ -179,18 +179,17 @@ struct ladds_ctx {
 
   StepState *d_step = nullptr;  // graph replay position (ladds_config.step_graph)
 
-  // Overlapped simulation.  The eager loop cycles through three buffers: step k+2's
-  // propagation (`stream`) runs beside step k+1's partition (`stream3`) and step k's pair
-  // scan (`stream2`).  The step graphs (ladds_config.step_graph) alternate buffers 0 and 1,
+  // Overlapped simulation.  The eager loop cycles through three buffers (four when the pair
+  // scans alternate over `stream2` and `stream4`): step k+2's propagation (`stream`) runs
+  // beside step k+1's partition (`stream3`) and step k's pair scan.  The step graphs (ladds_config.step_graph) alternate buffers 0 and 1,
   // the partition on `stream`.  Buffer 0 is the context's own arrays (`d`, `slot`, `rec`,
-  // `tps`, `scratch`, `arena_top`); pipe[1] and pipe[2] hold the others.
+  // `tps`, `scratch`, `arena_top`); pipe[1] .. pipe[3] hold the others.
   hipStream_t stream2 = nullptr, stream3 = nullptr;
   hipStream_t stream4 = nullptr;   // second pair-scan stream (pair_streams 2)
   hipEvent_t ev_join4 = nullptr;   // end of ladds_simulate: join of the two pair-scan streams
-  hipEvent_t ev_prop[3] = {nullptr, nullptr, nullptr}, ev_part[3] = {nullptr, nullptr, nullptr},
-             ev_pair[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_prop[4] = {}, ev_part[4] = {}, ev_pair[4] = {};
   hipEvent_t ev_join = nullptr;  // end of ladds_simulate: stream waits for stream2
-  PipeBuffer pipe[3];
+  PipeBuffer pipe[4];
 
   // Staging for the output records (ladds_particle_records, ladds_log_records).
   void *xbuf = nullptr;
@@ -857,9 +856,9 @@ int launch_pairs_of(ladds_ctx *ctx, int b, double dt_interp, bool digest, uint64
 constexpr size_t kPipelineStageBytes = 32 * 1024;
 
 // One iteration of the eager overlapped loop: propagate buffer `cur` into nxt = cur + 1
-// (mod 3) on `stream` and, when `detect`, partition nxt on `stream3` once the propagation is
-// done; its pair scan (issue_scan, on `stream2`) is queued by the next iteration.  Buffer
-// nxt was last read by the partition and the pair scan of the step three iterations back
+// (mod nbuf) on `stream` and, when `detect`, partition nxt on `stream3` once the propagation
+// is done; its pair scan (issue_scan) is queued by the next iteration.  Buffer nxt was last
+// read by the partition and the pair scan of the step nbuf iterations back
 // (`pending_next`): the propagation waits for that pair scan, which itself followed its
 // partition.  So step k+2's propagation, step k+1's partition and step k's pair scan can
 // all run at once, and the partition (bound by HBM traffic) no longer holds up the next
@@ -869,10 +868,10 @@ constexpr size_t kPipelineStageBytes = 32 * 1024;
 // propagation, partition and pair scan all go on `stream2` in order, with no hop between
 // streams on the way to the first pair scan; the next propagation, on `stream`, waits for
 // this one only.
-int overlapped_iteration(ladds_ctx *ctx, int cur, bool pending_next, size_t env_offset, int nsteps, bool detect,
-                         double burnup_r2, double dt_interp, bool digest, uint64_t iteration, uint64_t spawn_span,
-                         bool fill) {
-  const int nxt = (cur + 1) % 3;
+int overlapped_iteration(ladds_ctx *ctx, int nbuf, int cur, bool pending_next, size_t env_offset, int nsteps,
+                         bool detect, double burnup_r2, double dt_interp, bool digest, uint64_t iteration,
+                         uint64_t spawn_span, bool fill) {
+  const int nxt = (cur + 1) % nbuf;
   const size_t stage = ctx->form == kFormFp64 ? 0 : kPipelineStageBytes;
   if (fill && detect) {
     int rc = launch_prop_into(ctx, cur, nxt, env_offset, nsteps, true, burnup_r2, nullptr, 0, ctx->stream2);
@@ -1069,7 +1068,7 @@ int ladds_create(const ladds_config *cfg, ladds_ctx **out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream4, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join4, hipEventDisableTiming);
-  for (int b = 0; b < 3 && e == hipSuccess; ++b) {
+  for (int b = 0; b < 4 && e == hipSuccess; ++b) {
     e = hipEventCreateWithFlags(&ctx->ev_prop[b], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_part[b], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_pair[b], hipEventDisableTiming);
@@ -1125,7 +1124,7 @@ int ladds_destroy(ladds_ctx *ctx) {
     hipFree(pb.arena_top);
     hipFree(pb.scratch);
   }
-  for (int b = 0; b < 3; ++b) {
+  for (int b = 0; b < 4; ++b) {
     if (ctx->ev_prop[b]) hipEventDestroy(ctx->ev_prop[b]);
     if (ctx->ev_part[b]) hipEventDestroy(ctx->ev_part[b]);
     if (ctx->ev_pair[b]) hipEventDestroy(ctx->ev_pair[b]);
@@ -1718,13 +1717,19 @@ int ladds_autotune(ladds_ctx *ctx, double dt_interp, int repetitions, ladds_tuni
   // cell size that ties at the current bucket count is kept although it loses with more
   // buckets (measured on the 256-scenario ensemble: 12.6 km cells with 512 buckets per
   // plane 6% faster than the 10.1 or 15.2 km cells with 128 that the passes stopped at);
-  // last, with the simulate objective only, 12: one or two pair-scan streams
+  // last, with the simulate objective only, 12: one or two pair-scan streams; then 1 and 9
+  // again
   const int dims[] = {0, 1, 9, 2, 3, 4, 5, 6, 7, 10, 8};
   constexpr int kDims = (int)(sizeof(dims) / sizeof(dims[0]));
   std::vector<int> order(dims, dims + kDims);
   if (ensemble) order.insert(order.end(), dims, dims + kDims);
   order.push_back(11);
   if (ctx->cfg.tune_simulate) order.push_back(12);  // (a single detection cannot see it)
+  // workgroup size and bucket shape once more, after the shapes they interact with: two
+  // full runs at 1M objects (split, simulate objective) stopped at 256 threads with 2x1
+  // buckets (66.7 us per step) and at 512 threads with 2x2 (68.3 us)
+  order.push_back(1);
+  order.push_back(9);
   for (const int dim : order) {
     if (dim == 8 && !ensemble) continue;
     std::vector<Cand> cands;
@@ -2028,9 +2033,9 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     if (rc != LADDS_OK) return rc;
   }
   // Step graphs (ladds_config.step_graph, detection every step): see capture_step_graph;
-  // they alternate buffers 0 and 1.  Otherwise the eager loop cycles through three buffers,
-  // overlapping propagation (stream), partition (stream3) and pair scan (stream2) of
-  // consecutive steps: see overlapped_iteration.
+  // they alternate buffers 0 and 1.  Otherwise the eager loop cycles through three or four
+  // buffers, overlapping propagation (stream), partition (stream3) and pair scans (stream2,
+  // and stream4 with pair_streams 2) of consecutive steps: see overlapped_iteration.
   const bool use_graph = ctx->cfg.step_graph != 0 && tpcd == 1 && ctx->variant == kVariantPlaneTiles &&
                          !ctx->profiling && ctx->n > 0 && nsteps > 0;
   const bool use_overlap = !use_graph && ctx->variant == kVariantPlaneTiles && ctx->n > 0 && nsteps > 0;
@@ -2059,13 +2064,18 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_spawn, &ctx->h_rb->spawn, sizeof(SpawnState), hipMemcpyHostToDevice, head));
   }
   int cur = 0;                              // state buffer holding the latest state
-  bool pending[3] = {false, false, false};  // a pair scan reading buffer b may still run on stream2
+  bool pending[4] = {false, false, false, false};  // a pair scan reading buffer b may still run
   bool fill = use_overlap;                  // the next overlapped iteration is the call's first
   PendingScan scan;                         // pair scan still to queue (overlapped loop)
   const bool two_streams = !spawn && pair_streams(ctx) == 2;
+  // With two pair-scan streams a buffer's pair scan can still run beside the next step's,
+  // so the propagation that reuses it comes one step later: four buffers.  Measured at 1M
+  // objects (two streams): FP64 96.6 -> 93.3 us per step; split 69.7 -> 69.6 (20-step calls
+  // 77.3 -> 77.7, within the spread of repeated runs).
+  const int nbuf = two_streams ? 4 : 3;
   bool alt = false, used4 = false, last4 = false;  // next scan on stream4; stream4 used; last scan there
   if (use_overlap || use_graph) {
-    const int rc = ensure_overlap_buffers(ctx, use_graph ? 2 : 3);
+    const int rc = ensure_overlap_buffers(ctx, use_graph ? 2 : nbuf);
     if (rc != LADDS_OK) return rc;
   }
   hipGraphExec_t graphs[2] = {nullptr, nullptr};
@@ -2137,11 +2147,11 @@ int ladds_simulate(ladds_ctx *ctx, uint64_t first_iteration, int nsteps, int tpc
       if (detect) next_acc = spawn ? it + 1 : next_acc;
       int rc;
       if (use_overlap) {
-        const int nxt = (cur + 1) % 3;
+        const int nxt = (cur + 1) % nbuf;
         // the prologue ran on stream2: order this propagation after it
         rc = (fill && !detect) ? follow(ctx, ctx->ev_join, ctx->stream2, ctx->stream) : LADDS_OK;
         if (rc == LADDS_OK)
-          rc = overlapped_iteration(ctx, cur, pending[nxt], env_off + (size_t)s, k, detect, burnup_r2,
+          rc = overlapped_iteration(ctx, nbuf, cur, pending[nxt], env_off + (size_t)s, k, detect, burnup_r2,
                                     ctx->cfg.dt * (double)tpcd, digest, it, span, fill);
         // the previous iteration's pair scan, then this one's (the first iteration's ran)
         if (rc == LADDS_OK && scan.valid) {

@@ -316,15 +316,15 @@ def _d2_bits(x):
     return int(np.array(x, np.float64).view(np.int64))
 
 
-@pytest.mark.parametrize("steps,tpcd", [(30, 1), (31, 1), (32, 1), (29, 2), (40, 3)])
+@pytest.mark.parametrize("steps,tpcd", [(30, 1), (31, 1), (32, 1), (33, 1), (29, 2), (40, 3)])
 @pytest.mark.parametrize("precision", [la.FP64, la.SPLIT])
 def test_simulate_pipeline_equals_step_by_step(steps, tpcd, precision):
     """ladds_simulate's eager loop cycles through three buffers, with the propagation of step
     k+2, the partition of step k+1 and the pair scan of step k in flight together.  Its pair
     log (iteration, ids, d^2 bits, evaded) and its final state equal, bit for bit, those of
     the same steps run one call at a time (ladds_propagate, then ladds_conjunctions at each
-    detection iteration).  The step counts leave the final state in each of the three
-    buffers; detection every 1, 2 and 3 iterations."""
+    detection iteration).  The step counts leave the final state in each of the buffers
+    (three, or four with two pair-scan streams); detection every 1, 2 and 3 iterations."""
     pop = _cluster_population(19)
     cfg = dict(dt=10.0, components=la.ALL_COMPONENTS, precision=precision, cutoff_km=10.0,
                threshold_mode=la.THRESHOLD_ABSOLUTE, threshold_km=1.0, log_capacity=1 << 22)
