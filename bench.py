@@ -525,7 +525,11 @@ def main():
             # its bytes are far below what HBM allows: the limit is elsewhere
             out["note"] = ("bound by instruction issue and LDS latency in the neighbour scan, not HBM: "
                            "waves wait ~55% of their lifetime, VALU busy about half the kernel "
-                           "(profiles/pair_kernel_counters_1M.md)")
+                           "(profiles/pair_kernel_counters_1M.md). In the loop the pair scans of "
+                           "consecutive steps alternate over two streams and overlap; the events "
+                           "around a launch also hold the time its workgroups wait for CU slots "
+                           "the previous scan still uses (rocprofv3 start-to-end: ~58 us split, "
+                           "profiles/kernels_1M_full_model.md; alone: 47.2 us)")
         flops = MEASURED_FLOPS_PER_OBJECT.get((name, form))
         if flops:
             f64 = flops[0] * a.objects / (k["avg_us"] * 1e-6) / 1e12
