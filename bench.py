@@ -148,7 +148,9 @@ def run_form(la, pop, precision, steps, warmup, profile_steps, autotune=True, pi
         eng.set_tuning(pinned)
         tuning = dict(pinned)
     else:
-        tuning = eng.autotune(repetitions=3) if autotune else None
+        # best of 5 bursts per shape: shapes within ~2% of each other in step time are ranked
+        # on less noise (tuning runs before the timed region)
+        tuning = eng.autotune(repetitions=5) if autotune else None
     if warmup:
         eng.simulate(warmup, first_iteration=0, tpcd=1, min_altitude_km=MIN_ALT)
     eng.clear_log()
