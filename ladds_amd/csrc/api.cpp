@@ -849,11 +849,13 @@ int launch_pairs_of(ladds_ctx *ctx, int b, double dt_interp, bool digest, uint64
 
 // LDS window of the overlapped loop's partitions of FP32 records.  A workgroup staging a
 // whole 8,192-record tile takes 136 KB of LDS and waits for a CU the pair kernel has left;
-// staging 32 KB per pass (four passes) it runs beside three split pair workgroups.  Measured
-// at 1M objects: 1.41e10 -> 1.47e10 object-steps/s (16, 48 and 64 KB: 1.45-1.46e10).  FP64
+// staging part of it per pass it runs beside pair workgroups.  Measured at 1M objects with
+// one pair-scan stream and three buffers: 1.41e10 -> 1.47e10 object-steps/s with 32 KB (16,
+// 48 and 64 KB: 1.45-1.46e10); with two pair-scan streams and four buffers, 64 KB (two
+// passes): 70.9 -> 69.7 us per step, 20-step calls 80.4 -> 78.2 (16 KB: slower).  FP64
 // records keep the whole tile: two 80 KB FP64 pair workgroups leave no such room on a CU, and
 // the extra passes only cost (1.02e10 -> 0.98e10 with 32 or 64 KB).
-constexpr size_t kPipelineStageBytes = 32 * 1024;
+constexpr size_t kPipelineStageBytes = 64 * 1024;
 
 // One iteration of the eager overlapped loop: propagate buffer `cur` into nxt = cur + 1
 // (mod nbuf) on `stream` and, when `detect`, partition nxt on `stream3` once the propagation
