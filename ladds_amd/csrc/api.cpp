@@ -847,14 +847,15 @@ int launch_pairs_of(ladds_ctx *ctx, int b, double dt_interp, bool digest, uint64
   return LADDS_OK;
 }
 
-// LDS window of the overlapped loop's partitions of FP32 records.  A workgroup staging a
+// LDS window of the overlapped loop's partitions.  A workgroup staging a
 // whole 8,192-record tile takes 136 KB of LDS and waits for a CU the pair kernel has left;
 // staging part of it per pass it runs beside pair workgroups.  Measured at 1M objects with
 // one pair-scan stream and three buffers: 1.41e10 -> 1.47e10 object-steps/s with 32 KB (16,
 // 48 and 64 KB: 1.45-1.46e10); with two pair-scan streams and four buffers, 64 KB (two
 // passes): 70.9 -> 69.7 us per step, 20-step calls 80.4 -> 78.2 (16 KB: slower).  FP64
-// records keep the whole tile: two 80 KB FP64 pair workgroups leave no such room on a CU, and
-// the extra passes only cost (1.02e10 -> 0.98e10 with 32 or 64 KB).
+// records: with one pair-scan stream the passes only cost (1.02e10 -> 0.98e10 with 32 or
+// 64 KB against the whole tile), with two streams and four buffers 64 KB gains (93.7 -> 92.3
+// us per step, 20-step calls 103.9 -> 102.0).
 constexpr size_t kPipelineStageBytes = 64 * 1024;
 
 // One iteration of the eager overlapped loop: propagate buffer `cur` into nxt = cur + 1
@@ -874,7 +875,7 @@ int overlapped_iteration(ladds_ctx *ctx, int nbuf, int cur, bool pending_next, s
                          bool detect, double burnup_r2, double dt_interp, bool digest, uint64_t iteration,
                          uint64_t spawn_span, bool fill) {
   const int nxt = (cur + 1) % nbuf;
-  const size_t stage = ctx->form == kFormFp64 ? 0 : kPipelineStageBytes;
+  const size_t stage = kPipelineStageBytes;
   if (fill && detect) {
     int rc = launch_prop_into(ctx, cur, nxt, env_offset, nsteps, true, burnup_r2, nullptr, 0, ctx->stream2);
     if (rc == LADDS_OK) rc = follow(ctx, ctx->ev_prop[nxt], ctx->stream2, ctx->stream);
